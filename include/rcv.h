@@ -1,0 +1,228 @@
+/*
+ * rcv.h -- C ABI of librcv.so, the MI355X (gfx950) implementation of the
+ * recursive-control-variate inverse-rendering hot path.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: RCV_OK (0) or a negative
+ *     RCV_E_* code. rcv_last_error() returns a thread-local message for the
+ *     last failing call on the calling thread.
+ *   - The caller allocates every device buffer (float pointers below are HIP
+ *     device pointers unless stated otherwise); the library never frees caller
+ *     memory. Host arrays (geometry, transforms) are copied during the call.
+ *   - `stream` is a hipStream_t passed as void* (NULL = legacy default stream).
+ *     All device work is enqueued on it; no call synchronises the host unless
+ *     stated.
+ *   - Scene handles are thread-compatible (one thread at a time); a CV state is
+ *     not re-entrant: two rcv_cv_step calls on one state must be ordered on one
+ *     stream.
+ *   - Images are row-major (height, width, 3) float32. Textures are
+ *     (height, width, channels); grids are (depth, height, width, channels).
+ */
+#ifndef RCV_H
+#define RCV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RCV_OK 0
+#define RCV_E_INVALID (-1)   /* bad argument / shape / id */
+#define RCV_E_STATE (-2)     /* call not valid in the object's current state */
+#define RCV_E_HIP (-3)       /* a HIP runtime call failed */
+#define RCV_E_SCENE (-4)     /* scene check failed (see plugins/twostatepath.py:152-166) */
+#define RCV_E_NOMEM (-5)
+
+#define RCV_STATE_OLD 0
+#define RCV_STATE_NEW 1
+
+/* parameter kinds */
+#define RCV_PARAM_TEXTURE 0 /* (ny, nx, channels) bitmap, bilinear, repeat */
+#define RCV_PARAM_GRID 1    /* (nz, ny, nx, channels) grid, trilinear, clamp */
+
+/* bsdf types */
+#define RCV_BSDF_DIFFUSE 0
+#define RCV_BSDF_NULL 1
+
+/* emitter types */
+#define RCV_EMITTER_AREA 0     /* attached to a mesh shape */
+#define RCV_EMITTER_CONSTANT 1 /* constant environment */
+
+/* integrator types */
+#define RCV_INTEGRATOR_PATH 0   /* twostateprb / prb  (surfaces)  */
+#define RCV_INTEGRATOR_VOLPATH 1 /* twostateprbvolpath / prbvolpath */
+
+/* loss kinds */
+#define RCV_LOSS_L1 0
+#define RCV_LOSS_L2 1
+
+typedef struct rcv_scene rcv_scene;
+
+/* Integrator properties (plugins/twostatepath.py:106-118,147-150;
+ * plugins/twostatevolpath.py:20-32,62-65). */
+typedef struct rcv_integrator_desc {
+  int32_t type;          /* RCV_INTEGRATOR_* */
+  int32_t max_depth;     /* -1 = unbounded */
+  int32_t rr_depth;
+  int32_t hide_emitters; /* 0/1 */
+  int32_t samples_per_thread; /* launch shape: camera samples traced serially
+                                 per thread (1..spp, divides spp); 0 = auto.
+                                 Does not change results beyond float
+                                 summation order of the pixel average. */
+  int32_t block_size;    /* threads per block, 64..1024 multiple of 64; 0 = auto */
+} rcv_integrator_desc;
+
+/* Heterogeneous medium with an axis-aligned grid box (mitsuba 'heterogeneous'
+ * with a 'gridvolume' sigma_t, as used by experiments.py:52-69). */
+typedef struct rcv_medium_desc {
+  int32_t sigma_t_param;   /* RCV_PARAM_GRID id (1 channel) */
+  int32_t albedo_param;    /* RCV_PARAM_GRID id (3 channels) or -1 */
+  float albedo_rgb[3];     /* used when albedo_param < 0 */
+  float scale;             /* sigma_t = scale * grid */
+  float bbox_min[3];       /* world-space box the grid spans */
+  float bbox_max[3];
+  int32_t sample_emitters; /* NEE inside the medium (default 1) */
+  int32_t majorant_factor; /* 0 = one global majorant; k>0 = supergrid with
+                              cells of k^3 voxels traversed by DDA
+                              (plugins/twostatemedium.py:25-70) */
+} rcv_medium_desc;
+
+/* ---------------------------------------------------------------- library */
+int rcv_version(void);
+const char *rcv_last_error(void);
+
+/* ------------------------------------------------------------------ scene
+ * Replaces the scene assembled by mi.load_dict for the hot path: shapes,
+ * BSDFs (incl. the 'twostate' adapter, plugins/twostatebsdf.py:4-56), media
+ * (incl. 'twostatemedium', plugins/twostatemedium.py:5-200), emitters and the
+ * (batch) sensor. Build with rcv_scene_add_*, then rcv_scene_commit. */
+int rcv_scene_create(rcv_scene **out);
+int rcv_scene_destroy(rcv_scene *scene);
+
+/* Declares a differentiable parameter (texture or grid). */
+int rcv_scene_add_param(rcv_scene *scene, int32_t kind, int32_t nx, int32_t ny,
+                        int32_t nz, int32_t channels, int32_t *out_id);
+/* Binds the device data of a parameter for one state (RCV_STATE_OLD/NEW).
+ * Replaces TwoState*.parameters_changed (plugins/twostatebsdf.py:46-56,
+ * plugins/twostatemedium.py:189-200): instead of copying old<-new<-incoming
+ * the host rebinds pointers. For grids bound to a medium, the state's maximum
+ * (majorant, plugins/twostatemedium.py:73-75) is reduced on `stream`. */
+int rcv_scene_bind_param(rcv_scene *scene, int32_t param, int32_t state,
+                         const float *device_data, void *stream);
+
+int rcv_scene_add_bsdf(rcv_scene *scene, int32_t type, const float rgb[3],
+                       int32_t texture_param, int32_t *out_id);
+int rcv_scene_add_medium(rcv_scene *scene, const rcv_medium_desc *desc,
+                         int32_t *out_id);
+int rcv_scene_add_emitter(rcv_scene *scene, int32_t type, const float rgb[3],
+                          int32_t *out_id);
+/* Triangle mesh; `uv` may be NULL. interior/exterior are medium ids or -1;
+ * `emitter` is an RCV_EMITTER_AREA id or -1. */
+int rcv_scene_add_mesh(rcv_scene *scene, const float *vertices, const float *uv,
+                       int32_t n_vertices, const int32_t *triangles,
+                       int32_t n_triangles, int32_t bsdf, int32_t emitter,
+                       int32_t interior, int32_t exterior, int32_t *out_id);
+int rcv_scene_add_sphere(rcv_scene *scene, const float center[3], float radius,
+                         int32_t bsdf, int32_t interior, int32_t exterior,
+                         int32_t *out_id);
+/* A (batch) sensor: n_views perspective cameras side by side on one film of
+ * film_width x film_height; view k covers columns [k*w/n, (k+1)*w/n).
+ * to_world: n_views row-major 4x4 camera-to-world matrices (columns: left,
+ * up, forward, origin); fov_x: horizontal field of view in degrees per view.
+ * (mitsuba 'batch' + 'perspective', experiments.py:28-50.) */
+int rcv_scene_add_sensor(rcv_scene *scene, int32_t film_width,
+                         int32_t film_height, int32_t n_views,
+                         const float *to_world, const float *fov_x,
+                         int32_t *out_id);
+/* Builds the BVH and uploads geometry. No add_* call is valid afterwards. */
+int rcv_scene_commit(rcv_scene *scene, void *stream);
+/* check_scene of the two-state integrators (plugins/twostatepath.py:152-166,
+ * plugins/twostatevolpath.py:67-78): returns RCV_E_SCENE with a message if
+ * the scene lacks the required two-state adapter. `twostate_flags` bit0 =
+ * bsdf i is a two-state adapter (array of n_bsdf), same for media. */
+int rcv_scene_check_twostate(const rcv_scene *scene, int32_t integrator_type,
+                             const int32_t *bsdf_twostate, int32_t n_bsdf,
+                             const int32_t *medium_twostate, int32_t n_media);
+
+/* ----------------------------------------------------------------- render
+ * Plain render of the NEW state: integrator.render(scene, sensor, seed, spp)
+ * of the upstream prb / prbvolpath base class (reached from
+ * plugins/metaintegrator.py:87-88). `img`: (film_h, film_w, 3). */
+int rcv_render(const rcv_scene *scene, const rcv_integrator_desc *integrator,
+               int32_t sensor, uint32_t seed, int32_t spp, float *img,
+               void *stream);
+/* Two-state render: render_twostates (plugins/twostatepath.py:179-223,
+ * plugins/twostatevolpath.py:91-138). One path per sample carries both
+ * throughputs; img_old = H, img_new = F. */
+int rcv_render_twostates(const rcv_scene *scene,
+                         const rcv_integrator_desc *integrator, int32_t sensor,
+                         uint32_t seed, int32_t spp, float *img_old,
+                         float *img_new, void *stream);
+/* Adjoint: render_backward (plugins/cv_integrator.py:87-94, inherited PRB /
+ * PRBVolpath adjoint on the NEW state). grad_img: (film_h, film_w, 3) =
+ * dLoss/dimage. Gradients are ACCUMULATED (+=) into grad_ptrs[i], which has
+ * the shape of parameter param_ids[i]. Path replay: nothing is taped. */
+int rcv_render_backward(const rcv_scene *scene,
+                        const rcv_integrator_desc *integrator, int32_t sensor,
+                        uint32_t seed, int32_t spp, const float *grad_img,
+                        int32_t n_params, const int32_t *param_ids,
+                        float *const *grad_ptrs, void *stream);
+/* Device event counters of the last render on this scene (tentative
+ * collisions, NEE ratio-tracking steps, surface hits, paths): copies
+ * `n` int64 counters to host memory `out` (synchronises `stream`). Counting
+ * is enabled by rcv_scene_set_counting(scene, 1). */
+int rcv_scene_set_counting(rcv_scene *scene, int32_t enabled);
+int rcv_scene_read_counters(const rcv_scene *scene, int64_t *out, int32_t n,
+                            void *stream);
+
+/* --------------------------------------------------------------- CV state
+ * Per-pixel control-variate state of CVIntegrator / MetaIntegrator
+ * (plugins/cv_integrator.py:32-45, plugins/welford.py:44-57): eight float32
+ * arrays of n_elem elements and a device int32 counter block
+ * counters[0] = it, counters[1] = n (StatisticsEstimator.n), counters[2..3]
+ * library scratch; all zero at init_buffers. */
+typedef struct rcv_cv_state {
+  float *img;
+  float *v_n;
+  float *w_s;
+  float *mean_x;
+  float *mean_y;
+  float *var_x;
+  float *var_y;
+  float *cov;
+  int32_t *counters;
+} rcv_cv_state;
+
+/* One fused combine: weights, NaN/Inf guard, clamp, blend, statistics update
+ * and it += 1 (plugins/cv_integrator.py:60-76; pre_update ordering of
+ * plugins/metaintegrator.py:95-96,110-111). H = img_old, F = img_new. The
+ * counters are read and advanced on the device, so the call can be captured
+ * in a graph and replayed. */
+int rcv_cv_step(const rcv_cv_state *state, const float *H, const float *F,
+                int64_t n_elem, int32_t warmup, float beta1, float beta2,
+                int32_t pre_update, void *stream);
+/* WelfordVarianceEstimator.update (plugins/welford.py:27-33): mean/var arrays
+ * of n_elem, counter[0] = n. */
+int rcv_welford_update(float *mean, float *var, int32_t *counter,
+                       const float *x, int64_t n_elem, float beta1,
+                       float beta2, void *stream);
+
+/* ------------------------------------------------------- loss / optimizer */
+/* loss_out[0] = mean(|img-ref|) or mean((img-ref)^2) (utils.py:21-25) and
+ * grad_out = dloss/dimg (may be NULL). loss_out is a device float. */
+int rcv_loss(const float *img, const float *ref, int64_t n_elem, int32_t kind,
+             float *loss_out, float *grad_out, void *stream);
+/* Adam (mi.ad.Adam as used at optimize.py:90,127) fused with clamp
+ * (utils.py:172-175): param_out = clamp(param_in - lr_t*m/(sqrt(v)+eps)),
+ * with m, v updated in place; t is the 1-based step count. param_out may
+ * equal param_in. */
+int rcv_adam_step(float *param_out, const float *param_in, const float *grad,
+                  float *m, float *v, int64_t n_elem, int32_t t, float lr,
+                  float beta1, float beta2, float eps, float clamp_lo,
+                  float clamp_hi, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RCV_H */

@@ -1,0 +1,1200 @@
+// CPU oracle for the two-state renders (TEST INFRASTRUCTURE ONLY).
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// this library (oracle/_ref/... is never used; this file is the build's own
+// restatement), and only as the checker or the timed CPU baseline.
+//
+// Restates, per lane and in the reference's order of operations:
+//   TwoStatePRBVolpathIntegrator.sample_twostates     plugins/twostatevolpath.py:141-385
+//   TwoStatePRBVolpathIntegrator.sample_emitter_twostates  :387-479
+//   TwoStateMedium.sample_interaction_twostates        plugins/twostatemedium.py:15-107
+//   TwoStatePRBIntegrator.sample_twostates             plugins/twostatepath.py:225-364
+//   TwoStateBSDF eval/eval_old/sample                  plugins/twostatebsdf.py:25-41
+// plus the upstream services they call, with the same conventions as the
+// HIP library (PCG32 with TEA seeding per (seed, pixel*spp+sample), box
+// filter, cell-centred clamped trilinear grids, bilinear repeat textures,
+// uniform emitter selection, cosine-weighted diffuse sampling). Ray casts are
+// brute force over all primitives (no BVH), so the GPU traversal is checked
+// against an independent intersection routine.
+//
+// Templated on the real type: R = double is the high-precision reference,
+// R = float the same-precision restatement used for sample-by-sample
+// comparisons and as the timed CPU baseline. A plain render is the NEW image
+// of a two-state render whose two states are equal.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+extern "C" {
+typedef struct OracleScene {
+  int n_tris;
+  const float *tri_v;   // 9 per triangle
+  const float *tri_uv;  // 6 per triangle
+  const int *tri_shape;
+  int n_spheres;
+  const float *spheres;  // center xyz, radius
+  const int *sphere_shape;
+  int n_shapes;
+  const int *shapes;  // bsdf, emitter, interior, exterior
+  int n_bsdfs;
+  const int *bsdf_type;  // 0 diffuse, 1 null
+  const int *bsdf_tex;
+  const float *bsdf_rgb;
+  int n_emitters;
+  const int *em_type;  // 0 area, 1 constant
+  const int *em_shape;
+  const float *em_rgb;
+  int n_media;
+  const int *med_sigma;
+  const int *med_albedo;
+  const float *med_albedo_rgb;
+  const float *med_scale;
+  const float *med_lo;
+  const float *med_hi;
+  const int *med_sample_emitters;
+  int n_params;
+  const int *param_dims;  // nx, ny, nz, ch per param
+  const float *const *param_old;
+  const float *const *param_new;
+  int film_w, film_h, n_views;
+  const float *views;  // origin3 left3 up3 fwd3 tan_half  (13 per view)
+} OracleScene;
+
+typedef struct OracleIntegrator {
+  int type;  // 0 path (surfaces), 1 volpath
+  int max_depth;
+  int rr_depth;
+  int hide_emitters;
+} OracleIntegrator;
+}
+
+namespace oracle {
+
+// ------------------------------------------------------------------ RNG
+static inline void tea(uint32_t &v0, uint32_t &v1, int rounds, const uint32_t k[4]) {
+  uint32_t sum = 0;
+  for (int i = 0; i < rounds; ++i) {
+    sum += 0x9e3779b9u;
+    v0 += ((v1 << 4) + k[0]) ^ (v1 + sum) ^ ((v1 >> 5) + k[1]);
+    v1 += ((v0 << 4) + k[2]) ^ (v0 + sum) ^ ((v0 >> 5) + k[3]);
+  }
+}
+static const uint32_t kTeaKey[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+
+static inline uint64_t tea64(uint32_t a, uint32_t b) {
+  tea(a, b, 4, kTeaKey);
+  return (uint64_t(b) << 32) | a;
+}
+
+struct Pcg32 {
+  uint64_t state = 0, inc = 1;
+  uint32_t next_u32() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+  }
+  void seed(uint64_t initstate, uint64_t initseq) {
+    state = 0;
+    inc = (initseq << 1u) | 1u;
+    next_u32();
+    state += initstate;
+    next_u32();
+  }
+  // 23 random mantissa bits in [1, 2) minus 1: exact in float and double
+  double next_1d() {
+    uint32_t bits = (next_u32() >> 9) | 0x3f800000u;
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return double(f - 1.0f);
+  }
+};
+
+// ----------------------------------------------------------------- vectors
+template <typename R>
+struct V {
+  R x, y, z;
+  V() : x(0), y(0), z(0) {}
+  V(R a, R b, R c) : x(a), y(b), z(c) {}
+  explicit V(R a) : x(a), y(a), z(a) {}
+  V operator+(const V &o) const { return V(x + o.x, y + o.y, z + o.z); }
+  V operator-(const V &o) const { return V(x - o.x, y - o.y, z - o.z); }
+  V operator-() const { return V(-x, -y, -z); }
+  V operator*(const V &o) const { return V(x * o.x, y * o.y, z * o.z); }
+  V operator*(R s) const { return V(x * s, y * s, z * s); }
+  V operator/(R s) const { return V(x / s, y / s, z / s); }
+  R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <typename R>
+static inline R dot(const V<R> &a, const V<R> &b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename R>
+static inline V<R> cross(const V<R> &a, const V<R> &b) {
+  return V<R>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename R>
+static inline V<R> normalize(const V<R> &a) {
+  return a * (R(1) / std::sqrt(dot(a, a)));
+}
+template <typename R>
+static inline bool any_nz(const V<R> &a) {
+  return a.x != 0 || a.y != 0 || a.z != 0;
+}
+template <typename R>
+static inline R maxc(const V<R> &a) {
+  return std::max(a.x, std::max(a.y, a.z));
+}
+
+template <typename R>
+struct Consts {
+  static constexpr R pi = R(3.14159265358979323846);
+  static constexpr R inv_pi = R(0.31830988618379067154);
+  static constexpr R inv_4pi = R(0.07957747154594766788);
+  static constexpr R ray_eps = R(8.9407e-5);
+  static constexpr R shadow_eps = R(8.9407e-4);
+};
+
+template <typename R>
+static inline R mis_weight(R a, R b) {
+  // power heuristic (upstream mis_weight used at plugins/twostatepath.py:6)
+  R a2 = a * a;
+  return a > 0 ? a2 / (b * b + a2) : R(0);
+}
+
+template <typename R>
+static inline V<R> uniform_sphere(R u, R v) {
+  R z = R(1) - R(2) * u;
+  R r = std::sqrt(std::max(R(0), R(1) - z * z));
+  R phi = R(2) * Consts<R>::pi * v;
+  return V<R>(r * std::cos(phi), r * std::sin(phi), z);
+}
+
+template <typename R>
+static inline V<R> cosine_hemisphere(R u, R v) {
+  R a = R(2) * u - R(1), b = R(2) * v - R(1);
+  R x = 0, y = 0;
+  if (!(a == 0 && b == 0)) {
+    R r, phi;
+    if (std::fabs(a) > std::fabs(b)) {
+      r = a;
+      phi = (Consts<R>::pi * R(0.25)) * (b / a);
+    } else {
+      r = b;
+      phi = Consts<R>::pi * R(0.5) - (Consts<R>::pi * R(0.25)) * (a / b);
+    }
+    x = r * std::cos(phi);
+    y = r * std::sin(phi);
+  }
+  return V<R>(x, y, std::sqrt(std::max(R(0), R(1) - x * x - y * y)));
+}
+
+template <typename R>
+struct Frame {
+  V<R> s, t, n;
+  explicit Frame(const V<R> &nn) : n(nn) {
+    R sign = std::copysign(R(1), n.z);
+    R a = R(-1) / (sign + n.z);
+    R b = n.x * n.y * a;
+    s = V<R>(R(1) + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    t = V<R>(b, sign + n.y * n.y * a, -n.y);
+  }
+  V<R> to_local(const V<R> &v) const { return V<R>(dot(v, s), dot(v, t), dot(v, n)); }
+  V<R> to_world(const V<R> &v) const { return s * v.x + t * v.y + n * v.z; }
+};
+
+template <typename R>
+struct Ray {
+  V<R> o, d;
+  R maxt;
+};
+
+template <typename R>
+struct SI {
+  bool valid = false;
+  R t = std::numeric_limits<R>::infinity();
+  V<R> p, n;
+  R u = 0, v = 0;
+  int shape = -1;
+};
+
+template <typename R>
+static inline V<R> offset_p(const V<R> &p, const V<R> &n, const V<R> &d) {
+  R mag = (R(1) + maxc(V<R>(std::fabs(p.x), std::fabs(p.y), std::fabs(p.z)))) * Consts<R>::ray_eps;
+  mag = std::copysign(mag, dot(n, d));
+  return p + n * mag;
+}
+
+// ------------------------------------------------------------------ scene
+template <typename R>
+struct Scene {
+  const OracleScene &s;
+  std::vector<V<R>> tv0, te1, te2;
+  std::vector<R> em_inv_area;
+  std::vector<std::vector<int>> em_tris;
+  std::vector<std::vector<R>> em_cdf;
+  int env = -1;
+  V<R> bs_c;
+  R bs_r = 0;
+  std::vector<R> maj_old, maj_new;  // per param, unscaled max
+
+  explicit Scene(const OracleScene &sc) : s(sc) {
+    for (int t = 0; t < s.n_tris; ++t) {
+      const float *v = s.tri_v + 9 * t;
+      V<R> a(v[0], v[1], v[2]), b(v[3], v[4], v[5]), c(v[6], v[7], v[8]);
+      tv0.push_back(a);
+      te1.push_back(b - a);
+      te2.push_back(c - a);
+    }
+    em_inv_area.assign(s.n_emitters, 0);
+    em_tris.resize(s.n_emitters);
+    em_cdf.resize(s.n_emitters);
+    for (int e = 0; e < s.n_emitters; ++e) {
+      if (s.em_type[e] == 1) {
+        env = e;
+        continue;
+      }
+      double total = 0;
+      std::vector<double> areas;
+      for (int t = 0; t < s.n_tris; ++t) {
+        if (s.tri_shape[t] != s.em_shape[e]) continue;
+        const float *v = s.tri_v + 9 * t;
+        double e1[3] = {double(v[3]) - v[0], double(v[4]) - v[1], double(v[5]) - v[2]};
+        double e2[3] = {double(v[6]) - v[0], double(v[7]) - v[1], double(v[8]) - v[2]};
+        double c0 = e1[1] * e2[2] - e1[2] * e2[1], c1 = e1[2] * e2[0] - e1[0] * e2[2],
+               c2 = e1[0] * e2[1] - e1[1] * e2[0];
+        double a = 0.5 * std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        areas.push_back(a);
+        em_tris[e].push_back(t);
+        total += a;
+      }
+      double acc = 0;
+      for (double a : areas) {
+        acc += a;
+        em_cdf[e].push_back(R(float(acc / total)));
+      }
+      if (!em_cdf[e].empty()) em_cdf[e].back() = R(1);
+      em_inv_area[e] = R(float(1.0 / total));
+    }
+    // bounding sphere of the geometry
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int t = 0; t < s.n_tris; ++t)
+      for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], double(s.tri_v[9 * t + 3 * k + a]));
+          hi[a] = std::max(hi[a], double(s.tri_v[9 * t + 3 * k + a]));
+        }
+    for (int i = 0; i < s.n_spheres; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], double(s.spheres[4 * i + a]) - s.spheres[4 * i + 3]);
+        hi[a] = std::max(hi[a], double(s.spheres[4 * i + a]) + s.spheres[4 * i + 3]);
+      }
+    bs_c = V<R>(R(float(0.5 * (lo[0] + hi[0]))), R(float(0.5 * (lo[1] + hi[1]))), R(float(0.5 * (lo[2] + hi[2]))));
+    double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    bs_r = R(float(0.5 * std::sqrt(dx * dx + dy * dy + dz * dz)));
+    for (int p = 0; p < s.n_params; ++p) {
+      size_t n = size_t(s.param_dims[4 * p]) * s.param_dims[4 * p + 1] * s.param_dims[4 * p + 2] *
+                 s.param_dims[4 * p + 3];
+      float mo = -INFINITY, mn = -INFINITY;
+      for (size_t i = 0; i < n; ++i) {
+        mo = std::max(mo, s.param_old[p][i]);
+        mn = std::max(mn, s.param_new[p][i]);
+      }
+      maj_old.push_back(R(mo));
+      maj_new.push_back(R(mn));
+    }
+  }
+
+  // ray_intersect: closest hit over all primitives, brute force
+  SI<R> intersect(const Ray<R> &r) const {
+    SI<R> si;
+    R best = r.maxt;
+    int hit_t = -1, hit_s = -1;
+    R bu = 0, bv = 0;
+    for (int t = 0; t < s.n_tris; ++t) {
+      // Moller-Trumbore
+      V<R> pv = cross(r.d, te2[t]);
+      R det = dot(te1[t], pv);
+      if (det == 0) continue;
+      R inv = R(1) / det;
+      V<R> tvec = r.o - tv0[t];
+      R u = dot(tvec, pv) * inv;
+      if (!(u >= 0 && u <= 1)) continue;
+      V<R> qv = cross(tvec, te1[t]);
+      R v = dot(r.d, qv) * inv;
+      if (!(v >= 0 && u + v <= 1)) continue;
+      R tt = dot(te2[t], qv) * inv;
+      if (!(tt > 0 && tt < best)) continue;
+      best = tt;
+      hit_t = t;
+      hit_s = -1;
+      bu = u;
+      bv = v;
+    }
+    for (int i = 0; i < s.n_spheres; ++i) {
+      V<R> c(s.spheres[4 * i], s.spheres[4 * i + 1], s.spheres[4 * i + 2]);
+      R rad = s.spheres[4 * i + 3];
+      V<R> f = r.o - c;
+      R bp = -dot(f, r.d);
+      V<R> l = f + r.d * bp;
+      R disc = rad * rad - dot(l, l);
+      if (disc < 0) continue;
+      R cc = dot(f, f) - rad * rad;
+      R q = bp + std::copysign(std::sqrt(disc), bp);
+      R t0 = cc / q, t1 = q;
+      if (t0 > t1) std::swap(t0, t1);
+      R tt;
+      if (t0 > 0 && t0 < best)
+        tt = t0;
+      else if (t1 > 0 && t1 < best)
+        tt = t1;
+      else
+        continue;
+      best = tt;
+      hit_s = i;
+      hit_t = -1;
+    }
+    if (hit_t >= 0) {
+      si.valid = true;
+      si.t = best;
+      si.p = tv0[hit_t] + te1[hit_t] * bu + te2[hit_t] * bv;
+      si.n = normalize(cross(te1[hit_t], te2[hit_t]));
+      si.shape = s.tri_shape[hit_t];
+      const float *uv = s.tri_uv + 6 * hit_t;
+      R b0 = R(1) - bu - bv;
+      si.u = R(uv[0]) * b0 + R(uv[2]) * bu + R(uv[4]) * bv;
+      si.v = R(uv[1]) * b0 + R(uv[3]) * bu + R(uv[5]) * bv;
+    } else if (hit_s >= 0) {
+      si.valid = true;
+      si.t = best;
+      V<R> c(s.spheres[4 * hit_s], s.spheres[4 * hit_s + 1], s.spheres[4 * hit_s + 2]);
+      V<R> p = r.o + r.d * best;
+      si.n = normalize(p - c);
+      si.p = c + si.n * R(s.spheres[4 * hit_s + 3]);
+      si.shape = s.sphere_shape[hit_s];
+      si.u = (std::atan2(si.n.y, si.n.x) + Consts<R>::pi) * (R(0.5) * Consts<R>::inv_pi);
+      si.v = std::acos(std::min(R(1), std::max(R(-1), si.n.z))) * Consts<R>::inv_pi;
+    }
+    return si;
+  }
+
+  bool occluded(const Ray<R> &r) const { return intersect(r).valid; }
+
+  // -------------------------------------------------------- textures/grids
+  V<R> texture(int param, int state, R u, R v) const {
+    const int nx = s.param_dims[4 * param], ny = s.param_dims[4 * param + 1];
+    const float *d = state == 0 ? s.param_old[param] : s.param_new[param];
+    R x = u * R(nx) - R(0.5), y = v * R(ny) - R(0.5);
+    R fx = std::floor(x), fy = std::floor(y);
+    R tx = x - fx, ty = y - fy;
+    auto wrap = [](long i, long n) {
+      long m = i % n;
+      return m < 0 ? m + n : m;
+    };
+    long x0 = wrap(long(fx), nx), x1 = wrap(long(fx) + 1, nx);
+    long y0 = wrap(long(fy), ny), y1 = wrap(long(fy) + 1, ny);
+    long id[4] = {y0 * nx + x0, y0 * nx + x1, y1 * nx + x0, y1 * nx + x1};
+    R w[4] = {(R(1) - tx) * (R(1) - ty), tx * (R(1) - ty), (R(1) - tx) * ty, tx * ty};
+    V<R> r;
+    for (int k = 0; k < 4; ++k) r = r + V<R>(d[3 * id[k]], d[3 * id[k] + 1], d[3 * id[k] + 2]) * w[k];
+    return r;
+  }
+
+  // cell-centred trilinear lookup, clamped; returns channels 0..2 (1-channel grids in .x)
+  V<R> grid(int param, int state, int med, const V<R> &p) const {
+    const int nx = s.param_dims[4 * param], ny = s.param_dims[4 * param + 1], nz = s.param_dims[4 * param + 2],
+              ch = s.param_dims[4 * param + 3];
+    const float *d = state == 0 ? s.param_old[param] : s.param_new[param];
+    const float *lo = s.med_lo + 3 * med, *hi = s.med_hi + 3 * med;
+    R l[3] = {(p.x - R(lo[0])) / (R(hi[0]) - R(lo[0])), (p.y - R(lo[1])) / (R(hi[1]) - R(lo[1])),
+              (p.z - R(lo[2])) / (R(hi[2]) - R(lo[2]))};
+    int n[3] = {nx, ny, nz};
+    long i0[3], i1[3];
+    R t[3];
+    for (int a = 0; a < 3; ++a) {
+      R x = l[a] * R(n[a]) - R(0.5);
+      R f = std::floor(x);
+      t[a] = x - f;
+      long i = long(f);
+      i0[a] = std::min<long>(std::max<long>(i, 0), n[a] - 1);
+      i1[a] = std::min<long>(std::max<long>(i + 1, 0), n[a] - 1);
+    }
+    V<R> out;
+    for (int c = 0; c < 8; ++c) {
+      long ix = (c & 1) ? i1[0] : i0[0], iy = (c & 2) ? i1[1] : i0[1], iz = (c & 4) ? i1[2] : i0[2];
+      R w = ((c & 1) ? t[0] : R(1) - t[0]) * ((c & 2) ? t[1] : R(1) - t[1]) * ((c & 4) ? t[2] : R(1) - t[2]);
+      long idx = ((iz * ny + iy) * nx + ix) * ch;
+      if (ch == 1)
+        out = out + V<R>(d[idx], 0, 0) * w;
+      else
+        out = out + V<R>(d[idx], d[idx + 1], d[idx + 2]) * w;
+    }
+    return out;
+  }
+
+  // ---------------------------------------------------------------- emitters
+  struct DS {
+    V<R> p, n, d;
+    R dist = 0, pdf = 0;
+    int emitter = -1;
+  };
+
+  int si_emitter(const SI<R> &si) const { return si.valid ? s.shapes[4 * si.shape + 1] : env; }
+
+  V<R> em_rgb(int e) const { return V<R>(s.em_rgb[3 * e], s.em_rgb[3 * e + 1], s.em_rgb[3 * e + 2]); }
+
+  V<R> emitter_eval(int e, const SI<R> &si, const V<R> &ray_d) const {
+    if (e < 0) return V<R>();
+    if (s.em_type[e] == 1) return em_rgb(e);
+    return dot(si.n, ray_d) < 0 ? em_rgb(e) : V<R>();
+  }
+
+  R pdf_emitter_direction(const DS &ds) const {
+    if (ds.emitter < 0) return 0;
+    R sel = R(1) / R(s.n_emitters);
+    if (s.em_type[ds.emitter] == 1) return Consts<R>::inv_4pi * sel;
+    R dp = dot(ds.d, ds.n);
+    if (!(dp < 0)) return 0;
+    return em_inv_area[ds.emitter] * ds.dist * ds.dist / std::fabs(dp) * sel;
+  }
+
+  DS ds_from_hit(const SI<R> &si, const V<R> &ref_p, const V<R> &ray_d) const {
+    DS ds;
+    ds.emitter = si_emitter(si);
+    ds.p = si.p;
+    ds.n = si.n;
+    if (si.valid) {
+      V<R> dv = si.p - ref_p;
+      ds.dist = std::sqrt(dot(dv, dv));
+      ds.d = dv / ds.dist;
+    } else {
+      ds.d = ray_d;
+      ds.dist = std::numeric_limits<R>::infinity();
+    }
+    return ds;
+  }
+
+  V<R> sample_emitter_direction(const V<R> &ref, R u1, R u2, DS &ds) const {
+    ds = DS();
+    int ne = s.n_emitters;
+    if (ne == 0) return V<R>();
+    int idx = std::min(int(u1 * R(ne)), ne - 1);
+    u1 = u1 * R(ne) - R(idx);
+    R sel = R(1) / R(ne);
+    ds.emitter = idx;
+    if (s.em_type[idx] == 1) {
+      V<R> d = uniform_sphere(u1, u2);
+      V<R> off = ref - bs_c;
+      R radius = std::max(bs_r, std::sqrt(dot(off, off)));
+      ds.dist = R(2) * radius;
+      ds.d = d;
+      ds.p = ref + d * ds.dist;
+      ds.n = -d;
+      ds.pdf = Consts<R>::inv_4pi * sel;
+      return em_rgb(idx) * (R(4) * Consts<R>::pi * R(ne));
+    }
+    const auto &cdf = em_cdf[idx];
+    int k = int(std::lower_bound(cdf.begin(), cdf.end(), u1) - cdf.begin());
+    k = std::min<int>(k, int(cdf.size()) - 1);
+    R c0 = k > 0 ? cdf[k - 1] : R(0), c1 = cdf[k];
+    u1 = std::min(std::max((u1 - c0) / std::max(c1 - c0, R(1e-30)), R(0)), R(0.99999994f));
+    int t = em_tris[idx][k];
+    R su = std::sqrt(R(1) - u1);
+    R b1 = R(1) - su, b2 = su * u2;
+    ds.p = tv0[t] + te1[t] * b1 + te2[t] * b2;
+    ds.n = normalize(cross(te1[t], te2[t]));
+    V<R> dv = ds.p - ref;
+    R d2 = dot(dv, dv);
+    ds.dist = std::sqrt(d2);
+    ds.d = dv / ds.dist;
+    R dp = dot(ds.d, ds.n);
+    ds.pdf = em_inv_area[idx] * d2 / std::fabs(dp) * sel;
+    if (!(dp < 0) || !(ds.pdf > 0)) return V<R>();
+    return em_rgb(idx) / ds.pdf;
+  }
+
+  // ------------------------------------------------------------------ BSDF
+  V<R> reflectance(int bsdf, int state, R u, R v) const {
+    if (s.bsdf_tex[bsdf] < 0) return V<R>(s.bsdf_rgb[3 * bsdf], s.bsdf_rgb[3 * bsdf + 1], s.bsdf_rgb[3 * bsdf + 2]);
+    return texture(s.bsdf_tex[bsdf], state, u, v);
+  }
+  int bsdf_of(const SI<R> &si) const { return si.valid ? s.shapes[4 * si.shape] : -1; }
+  bool is_null(int b) const { return b >= 0 && s.bsdf_type[b] == 1; }
+  bool is_diffuse(int b) const { return b >= 0 && s.bsdf_type[b] == 0; }
+};
+
+template <typename R>
+static inline V<R> diffuse_eval(const V<R> &rho, const V<R> &wi, const V<R> &wo) {
+  if (!(wi.z > 0 && wo.z > 0)) return V<R>();
+  return rho * (Consts<R>::inv_pi * wo.z);
+}
+template <typename R>
+static inline R diffuse_pdf(const V<R> &wi, const V<R> &wo) {
+  if (!(wi.z > 0 && wo.z > 0)) return 0;
+  return Consts<R>::inv_pi * wo.z;
+}
+
+// ---------------------------------------------------------------- medium
+// TwoStateMedium.sample_interaction_twostates (plugins/twostatemedium.py:15-107)
+template <typename R>
+struct MEI {
+  bool valid = false;
+  R t = std::numeric_limits<R>::infinity();
+  R mint = 0, combined = 0;
+  V<R> p;
+  R sigma_t[2] = {0, 0}, sigma_n[2] = {0, 0};
+  V<R> sigma_s[2];
+};
+
+template <typename R>
+static MEI<R> sample_interaction_twostates(const Scene<R> &S, int med, const Ray<R> &ray, R sample) {
+  MEI<R> mei;
+  const float *lo = S.s.med_lo + 3 * med, *hi = S.s.med_hi + 3 * med;
+  // prepare_interaction_sampling: ray / medium bounding box
+  R t0 = -std::numeric_limits<R>::infinity(), t1 = std::numeric_limits<R>::infinity();
+  for (int a = 0; a < 3; ++a) {
+    R inv = R(1) / ray.d[a];
+    R ta = (R(lo[a]) - ray.o[a]) * inv, tb = (R(hi[a]) - ray.o[a]) * inv;
+    if (ta > tb) std::swap(ta, tb);
+    t0 = ta > t0 ? ta : t0;
+    t1 = tb < t1 ? tb : t1;
+  }
+  bool active = (t0 <= t1) && (std::isfinite(t0) || std::isfinite(t1));
+  R mint = active ? std::max(R(0), t0) : R(0);
+  R maxt = active ? std::min(ray.maxt, t1) : std::numeric_limits<R>::infinity();
+  const int sp = S.s.med_sigma[med];
+  const R scale = S.s.med_scale[med];
+  // global majorant max(maj_old, maj_new) (:71-75)
+  R combined = std::max(S.maj_old[sp] * scale, S.maj_new[sp] * scale);
+  active = active && combined > 0 && std::isfinite(combined);
+  R desired_tau = -std::log(R(1) - sample);
+  R sampled_t = mint + desired_tau / combined;
+  bool valid_mi = active && (sampled_t <= maxt);
+  mei.valid = valid_mi;
+  mei.t = valid_mi ? sampled_t : std::numeric_limits<R>::infinity();
+  mei.p = ray.o + ray.d * sampled_t;
+  mei.mint = mint;
+  mei.combined = combined;
+  for (int k = 0; k < 2; ++k) {
+    if (valid_mi) {
+      mei.sigma_t[k] = scale * S.grid(sp, k, med, mei.p).x;
+      V<R> alb = S.s.med_albedo[med] >= 0
+                     ? S.grid(S.s.med_albedo[med], k, med, mei.p)
+                     : V<R>(S.s.med_albedo_rgb[3 * med], S.s.med_albedo_rgb[3 * med + 1],
+                            S.s.med_albedo_rgb[3 * med + 2]);
+      mei.sigma_s[k] = alb * mei.sigma_t[k];
+    }
+    mei.sigma_n[k] = combined - mei.sigma_t[k];  // :96-97
+  }
+  return mei;
+}
+
+template <typename R>
+static inline int target_medium(const Scene<R> &S, const SI<R> &si, const V<R> &d) {
+  const int *sh = S.s.shapes + 4 * si.shape;
+  return dot(si.n, d) > 0 ? sh[3] : sh[2];
+}
+template <typename R>
+static inline bool is_medium_transition(const Scene<R> &S, const SI<R> &si) {
+  const int *sh = S.s.shapes + 4 * si.shape;
+  return sh[2] != sh[3];
+}
+
+template <typename R>
+static Ray<R> spawn_ray(const SI<R> &si, const V<R> &d) {
+  Ray<R> r;
+  r.o = offset_p(si.p, si.n, d);
+  r.d = d;
+  r.maxt = std::numeric_limits<R>::infinity();
+  return r;
+}
+
+// Path-replay adjoint state for the NEW state (upstream PRB / PRBVolpath
+// render_backward, reached through plugins/cv_integrator.py:87-94): the
+// image-space adjoint dL of this camera sample, the radiance Lrem still to be
+// gathered along the replayed path, and a double gradient buffer for one
+// density parameter.
+template <typename R>
+struct Adj {
+  V<R> dL, Lrem;
+  int param = -1;
+  std::vector<double> *g = nullptr;
+  double nee_scale = 0;  // dot(dL, NEE contribution) in an NEE gradient pass
+};
+
+// Adds value * trilinear weight to the 8 voxels of the lookup at p.
+template <typename R>
+static void grid_scatter(const Scene<R> &S, int param, int med, const V<R> &p, double value,
+                         std::vector<double> &g) {
+  const int nx = S.s.param_dims[4 * param], ny = S.s.param_dims[4 * param + 1], nz = S.s.param_dims[4 * param + 2];
+  const float *lo = S.s.med_lo + 3 * med, *hi = S.s.med_hi + 3 * med;
+  R l[3] = {(p.x - R(lo[0])) / (R(hi[0]) - R(lo[0])), (p.y - R(lo[1])) / (R(hi[1]) - R(lo[1])),
+            (p.z - R(lo[2])) / (R(hi[2]) - R(lo[2]))};
+  int n[3] = {nx, ny, nz};
+  long i0[3], i1[3];
+  R t[3];
+  for (int a = 0; a < 3; ++a) {
+    R x = l[a] * R(n[a]) - R(0.5);
+    R f = std::floor(x);
+    t[a] = x - f;
+    long i = long(f);
+    i0[a] = std::min<long>(std::max<long>(i, 0), n[a] - 1);
+    i1[a] = std::min<long>(std::max<long>(i + 1, 0), n[a] - 1);
+  }
+  for (int c = 0; c < 8; ++c) {
+    long ix = (c & 1) ? i1[0] : i0[0], iy = (c & 2) ? i1[1] : i0[1], iz = (c & 4) ? i1[2] : i0[2];
+    R w = ((c & 1) ? t[0] : R(1) - t[0]) * ((c & 2) ? t[1] : R(1) - t[1]) * ((c & 4) ? t[2] : R(1) - t[2]);
+    g[size_t((iz * ny + iy) * nx + ix)] += value * double(w);
+  }
+}
+
+// sample_emitter_twostates (plugins/twostatevolpath.py:387-479). With `gadj`
+// set, the pass also scatters d(log T_new)/d(density) * gadj->nee_scale.
+template <typename R>
+static void sample_emitter_twostates(const Scene<R> &S, const V<R> &ref_p, const V<R> &ref_n, bool ref_surface,
+                                     int medium, Pcg32 &rng, V<R> em[2], typename Scene<R>::DS &ds,
+                                     Adj<R> *gadj = nullptr) {
+  R u1 = R(rng.next_1d()), u2 = R(rng.next_1d());
+  V<R> emitter_val = S.sample_emitter_direction(ref_p, u1, u2, ds);
+  em[0] = em[1] = V<R>();
+  if (ds.pdf == 0) return;  // invalid: emitter_val = 0, lane inactive (:395-397)
+  Ray<R> ray;
+  ray.o = ref_surface ? offset_p(ref_p, ref_n, ds.d) : ref_p;
+  ray.d = ds.d;
+  R total_dist = 0;
+  SI<R> si;
+  bool needs_intersection = true;
+  R tr_old = 1, tr_new = 1;
+  bool active = true;
+  while (active) {
+    R remaining_dist = ds.dist * (R(1) - Consts<R>::shadow_eps) - total_dist;
+    ray.maxt = remaining_dist;
+    active = active && remaining_dist > 0;
+    if (!active) break;
+    if (needs_intersection) si = S.intersect(ray);
+    needs_intersection = false;
+    bool active_medium = medium >= 0;
+    bool active_surface = !active_medium;
+    MEI<R> mei;
+    if (active_medium) {
+      mei = sample_interaction_twostates(S, medium, ray, R(rng.next_1d()));
+      if (si.t < mei.t) {
+        mei.t = std::numeric_limits<R>::infinity();
+        mei.valid = false;
+      }
+    }
+    bool escaped_medium = active_medium && !mei.valid;
+    active_medium = active_medium && mei.valid;
+    R m_old = 1, m_new = 1;
+    if (active_medium) {
+      ray.o = mei.p;
+      si.t = si.t - mei.t;
+      m_old = mei.combined != 0 ? mei.sigma_n[0] / mei.combined : mei.sigma_n[0];
+      m_new = mei.combined != 0 ? mei.sigma_n[1] / mei.combined : mei.sigma_n[1];
+      if (gadj && S.s.med_sigma[medium] == gadj->param && mei.sigma_n[1] != 0)
+        grid_scatter(S, gadj->param, medium, mei.p,
+                     -gadj->nee_scale * double(S.s.med_scale[medium]) / double(mei.sigma_n[1]), *gadj->g);
+    }
+    active_surface = (active_surface || escaped_medium) && si.valid && !active_medium;
+    if (active_surface) {
+      R b = S.is_null(S.bsdf_of(si)) ? R(1) : R(0);  // eval_null_transmission
+      m_old = m_old * b;
+      m_new = m_new * b;
+    }
+    tr_old *= m_old;
+    tr_new *= m_new;
+    if (active_surface) {
+      ray = spawn_ray(si, ray.d);
+      needs_intersection = true;
+    }
+    active = (active_medium || active_surface) && (tr_new != 0 || tr_old != 0);
+    if (active) total_dist += active_medium ? mei.t : si.t;
+    if (active_surface && is_medium_transition(S, si)) medium = target_medium(S, si, ray.d);
+  }
+  em[0] = emitter_val * tr_old;
+  em[1] = emitter_val * tr_new;
+}
+
+// TwoStatePRBVolpathIntegrator.sample_twostates (plugins/twostatevolpath.py:141-385)
+template <typename R>
+static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &rng, Ray<R> ray, V<R> L[2],
+                              Adj<R> *adj = nullptr) {
+  using DS = typename Scene<R>::DS;
+  const R inf = std::numeric_limits<R>::infinity();
+  const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
+  V<R> L_old, L_new, thr_old(1), thr_new(1);
+  R eta = 1;
+  bool active = true;
+  SI<R> si;
+  bool needs_intersection = true;
+  V<R> last_p;
+  R last_pdf = 1;
+  int medium = -1;
+  int depth = 0;
+  bool specular_chain = true;
+  (void)rng.next_1d();  // colour channel (:171-173)
+  while (active) {
+    active = active && (any_nz(thr_new) || any_nz(thr_old));
+    if (!active) break;
+    V<R> tmax(std::max(thr_new.x, thr_old.x), std::max(thr_new.y, thr_old.y), std::max(thr_new.z, thr_old.z));
+    R q = std::min(maxc(tmax) * (eta * eta), R(0.99));
+    bool perform_rr = depth > I.rr_depth;
+    R urr = R(rng.next_1d());
+    active = active && ((urr < q) || !perform_rr);
+    if (!active) break;
+    if (perform_rr) {
+      thr_new = thr_new * (R(1) / q);
+      thr_old = thr_old * (R(1) / q);
+    }
+    bool active_medium = medium >= 0;
+    bool active_surface = !active_medium;
+    MEI<R> mei;
+    if (active_medium) {
+      R u = R(rng.next_1d());
+      mei = sample_interaction_twostates(S, medium, ray, u);
+      if (needs_intersection) si = S.intersect(ray);
+      needs_intersection = false;
+      if (si.t < mei.t) {
+        mei.t = inf;
+        mei.valid = false;
+      }
+    }
+    V<R> w_new(1), w_old(1);
+    bool escaped_medium = active_medium && !mei.valid;
+    active_medium = active_medium && mei.valid;
+    bool act_null = false, act_scatter = false;
+    if (active_medium) {
+      R majorant = mei.combined;
+      R p_new = mei.sigma_t[1] / majorant, p_old = mei.sigma_t[0] / majorant;
+      R scatter_prob = majorant != 0 ? (p_new + p_old) * R(0.5) : R(0);
+      act_null = R(rng.next_1d()) >= scatter_prob;
+      act_scatter = !act_null;
+      if (act_null) {
+        R den = mei.sigma_n[1] + mei.sigma_n[0];
+        w_new = w_new * (R(2) * mei.sigma_n[1] / den);
+        w_old = w_old * (R(2) * mei.sigma_n[0] / den);
+      }
+    }
+    if (act_scatter) {
+      depth += 1;
+      last_p = mei.p;
+    }
+    active = active && depth < max_depth;
+    act_scatter = act_scatter && active;
+    if (act_null) {
+      ray.o = mei.p;
+      si.t = si.t - mei.t;
+    }
+    if (act_scatter) {
+      R den = mei.sigma_t[1] + mei.sigma_t[0];
+      w_new = w_new * (mei.sigma_s[1] * R(2) / den);
+      w_old = w_old * (mei.sigma_s[0] * R(2) / den);
+    }
+    if (active_medium) {
+      thr_new = thr_new * w_new;
+      thr_old = thr_old * w_old;
+    }
+    if (adj && (act_null || act_scatter) && S.s.med_sigma[medium] == adj->param) {
+      // d log(weight_new)/d density times everything gathered after it:
+      // null: weight sigma_n/sigma_n(detached) -> -scale/sigma_n;
+      // real: weight sigma_s/sigma_t(detached) -> +scale/sigma_t.
+      const double lr = double(dot(adj->dL, adj->Lrem));
+      const double scale = S.s.med_scale[medium];
+      if (act_null) {
+        if (mei.sigma_n[1] != 0) grid_scatter(S, adj->param, medium, mei.p, -lr * scale / double(mei.sigma_n[1]), *adj->g);
+      } else {
+        grid_scatter(S, adj->param, medium, mei.p, lr * scale / double(mei.sigma_t[1]), *adj->g);
+      }
+    }
+    V<R> wo;
+    R phase_pdf = 0;
+    if (act_scatter) {
+      (void)rng.next_1d();
+      R a = R(rng.next_1d()), b = R(rng.next_1d());
+      wo = uniform_sphere(a, b);
+      phase_pdf = Consts<R>::inv_4pi;
+      ray.o = mei.p;
+      ray.d = wo;
+      ray.maxt = inf;
+      needs_intersection = true;
+      last_pdf = phase_pdf;
+    }
+    active_surface = active_surface || escaped_medium;
+    if (active_surface && needs_intersection) si = S.intersect(ray);
+    if (active_surface) {
+      int emitter = S.si_emitter(si);
+      bool active_e = emitter >= 0 && !(depth == 0 && I.hide_emitters);
+      if (active_e) {
+        bool count_direct = depth == 0 || specular_chain;
+        DS ds = S.ds_from_hit(si, last_p, ray.d);
+        R emitter_pdf = S.pdf_emitter_direction(ds);
+        V<R> emitted = S.emitter_eval(emitter, si, ray.d);
+        R mis_bsdf = mis_weight(last_pdf, emitter_pdf);
+        V<R> c_new = count_direct ? thr_new * emitted : thr_new * mis_bsdf * emitted;
+        L_new = L_new + c_new;
+        L_old = L_old + (count_direct ? thr_old * emitted : thr_old * mis_bsdf * emitted);
+        if (adj) adj->Lrem = adj->Lrem - c_new;
+      }
+    }
+    active_surface = active_surface && si.valid;
+    int bsdf = active_surface ? S.bsdf_of(si) : -1;
+    Frame<R> fr(active_surface ? si.n : V<R>(0, 0, 1));
+    V<R> wi = fr.to_local(-ray.d);
+    bool active_e_surface = active_surface && S.is_diffuse(bsdf) && (depth + 1 < max_depth);
+    bool sample_emitters = act_scatter && S.s.med_sample_emitters[medium] != 0;
+    specular_chain = specular_chain && !act_scatter;
+    specular_chain = specular_chain || (act_scatter && !sample_emitters);
+    bool active_e_medium = act_scatter && sample_emitters;
+    if (active_e_surface || active_e_medium) {
+      V<R> em[2];
+      DS ds;
+      V<R> ref_p = active_e_medium ? mei.p : si.p;
+      const Pcg32 rng_nee = rng;
+      sample_emitter_twostates(S, ref_p, si.n, active_e_surface, medium, rng, em, ds);
+      V<R> nw_new, nw_old;
+      R nee_pdf;
+      if (active_e_surface) {
+        V<R> wo_em = fr.to_local(ds.d);
+        nw_new = diffuse_eval(S.reflectance(bsdf, 1, si.u, si.v), wi, wo_em);
+        nw_old = diffuse_eval(S.reflectance(bsdf, 0, si.u, si.v), wi, wo_em);
+        nee_pdf = diffuse_pdf(wi, wo_em);
+      } else {
+        nw_new = nw_old = V<R>(Consts<R>::inv_4pi);
+        nee_pdf = Consts<R>::inv_4pi;
+      }
+      R mis_em = mis_weight(ds.pdf, nee_pdf);
+      V<R> c_new = thr_new * nw_new * mis_em * em[1];
+      L_new = L_new + c_new;
+      L_old = L_old + thr_old * nw_old * mis_em * em[0];
+      if (adj) {
+        // replay the same NEE random numbers, scattering d(log T)/d density
+        const double cs = double(dot(adj->dL, c_new));
+        if (cs != 0) {
+          Pcg32 r2 = rng_nee;
+          V<R> em2[2];
+          DS ds2;
+          Adj<R> g = *adj;
+          g.nee_scale = cs;
+          sample_emitter_twostates(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, &g);
+        }
+        adj->Lrem = adj->Lrem - c_new;
+      }
+    }
+    // BSDF sampling (:347-380)
+    bool bs_null = false;
+    R bs_pdf = 0;
+    V<R> bs_wo, bs_weight;
+    if (active_surface) {
+      (void)rng.next_1d();
+      R a = R(rng.next_1d()), b = R(rng.next_1d());
+      if (S.is_null(bsdf)) {
+        bs_wo = -wi;
+        bs_pdf = 1;
+        bs_weight = V<R>(1);
+        bs_null = true;
+      } else if (S.is_diffuse(bsdf) && wi.z > 0) {
+        bs_wo = cosine_hemisphere(a, b);
+        bs_pdf = diffuse_pdf(wi, bs_wo);
+        bs_weight = bs_pdf > 0 ? S.reflectance(bsdf, 1, si.u, si.v) : V<R>();
+      }
+      active_surface = bs_pdf > 0;
+    }
+    if (active_surface) {
+      V<R> w_old_b;
+      if (bs_null)
+        w_old_b = bs_weight;
+      else
+        w_old_b = diffuse_eval(S.reflectance(bsdf, 0, si.u, si.v), wi, bs_wo) / bs_pdf;
+      thr_old = thr_old * w_old_b;
+      thr_new = thr_new * bs_weight;
+      V<R> d = bs_null ? ray.d : fr.to_world(bs_wo);
+      ray = spawn_ray(si, d);
+      needs_intersection = true;
+      if (!bs_null) {
+        depth += 1;
+        last_p = si.p;
+        last_pdf = bs_pdf;
+        specular_chain = false;
+      }
+      if (is_medium_transition(S, si)) medium = target_medium(S, si, ray.d);
+    }
+    active = active && (active_surface || active_medium);
+  }
+  L[0] = L_old;
+  L[1] = L_new;
+}
+
+// TwoStatePRBIntegrator.sample_twostates (plugins/twostatepath.py:225-364)
+template <typename R>
+static void path_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &rng, Ray<R> ray, V<R> L[2]) {
+  using DS = typename Scene<R>::DS;
+  const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
+  V<R> L_old, L_new, b_old(1), b_new(1);
+  R eta = 1;
+  int depth = 0;
+  V<R> prev_p;
+  R prev_pdf = 1;
+  bool prev_delta = true;
+  bool active = true;
+  while (active) {
+    SI<R> si = S.intersect(ray);
+    {
+      DS ds = S.ds_from_hit(si, prev_p, ray.d);
+      R em_pdf = prev_delta ? R(0) : S.pdf_emitter_direction(ds);
+      R mis = mis_weight(prev_pdf, em_pdf);
+      V<R> le = S.emitter_eval(ds.emitter, si, ray.d);
+      L_old = L_old + b_old * mis * le;
+      L_new = L_new + b_new * mis * le;
+    }
+    bool active_next = (depth + 1 < max_depth) && si.valid;
+    int bsdf = S.bsdf_of(si);
+    bool smooth = S.is_diffuse(bsdf);
+    Frame<R> fr(si.valid ? si.n : V<R>(0, 0, 1));
+    V<R> wi = fr.to_local(-ray.d);
+    V<R> rho_new, rho_old;
+    if (active_next && smooth) {
+      rho_new = S.reflectance(bsdf, 1, si.u, si.v);
+      rho_old = S.reflectance(bsdf, 0, si.u, si.v);
+    }
+    {
+      R u1 = R(rng.next_1d()), u2 = R(rng.next_1d());
+      bool active_em = active_next && smooth;
+      if (active_em) {
+        DS ds;
+        V<R> em_w = S.sample_emitter_direction(si.p, u1, u2, ds);
+        active_em = ds.pdf != 0;
+        if (active_em && any_nz(em_w)) {
+          // shadow ray (Interaction::spawn_ray_to)
+          Ray<R> sr;
+          sr.o = offset_p(si.p, si.n, ds.p - si.p);
+          V<R> d = ds.p - sr.o;
+          R dist = std::sqrt(dot(d, d));
+          sr.d = d / dist;
+          sr.maxt = dist * (R(1) - Consts<R>::shadow_eps);
+          if (S.occluded(sr)) em_w = V<R>();
+        }
+        if (active_em && any_nz(em_w)) {
+          V<R> wo = fr.to_local(ds.d);
+          R pdf = diffuse_pdf(wi, wo);
+          R mis_em = mis_weight(ds.pdf, pdf);
+          L_old = L_old + b_old * mis_em * diffuse_eval(rho_old, wi, wo) * em_w;
+          L_new = L_new + b_new * mis_em * diffuse_eval(rho_new, wi, wo) * em_w;
+        }
+      }
+    }
+    {
+      (void)rng.next_1d();
+      R a = R(rng.next_1d()), b = R(rng.next_1d());
+      bool bs_null = false;
+      R bs_pdf = 0;
+      V<R> bs_wo(0, 0, 1), bs_weight;
+      if (active_next) {
+        if (S.is_null(bsdf)) {
+          bs_wo = -wi;
+          bs_pdf = 1;
+          bs_weight = V<R>(1);
+          bs_null = true;
+        } else if (smooth && wi.z > 0) {
+          bs_wo = cosine_hemisphere(a, b);
+          bs_pdf = diffuse_pdf(wi, bs_wo);
+          bs_weight = bs_pdf > 0 ? rho_new : V<R>();
+        }
+      }
+      prev_delta = bs_null;
+      if (si.valid) ray = spawn_ray(si, bs_null ? ray.d : fr.to_world(bs_wo));
+      V<R> w_old;
+      if (bs_null)
+        w_old = bs_weight;
+      else {
+        V<R> f_old = (active_next && smooth) ? diffuse_eval(rho_old, wi, bs_wo) : V<R>();
+        w_old = bs_pdf > 0 ? f_old / bs_pdf : V<R>();
+      }
+      b_old = b_old * w_old;
+      b_new = b_new * bs_weight;
+      prev_p = si.p;
+      prev_pdf = bs_pdf;
+    }
+    R b_max = maxc(b_new);
+    active_next = active_next && b_max != 0;
+    R rr_prob = std::min(b_max * (eta * eta), R(0.95));
+    bool rr_active = depth >= I.rr_depth;
+    if (rr_active) {
+      b_old = b_old * (R(1) / rr_prob);
+      b_new = b_new * (R(1) / rr_prob);
+    }
+    R urr = R(rng.next_1d());
+    active_next = active_next && (!rr_active || urr < rr_prob);
+    if (si.valid) depth += 1;
+    active = active_next;
+  }
+  L[0] = L_old;
+  L[1] = L_new;
+}
+
+template <typename R>
+static Ray<R> camera_ray(const OracleScene &s, int px, int py, Pcg32 &rng) {
+  int view_w = s.film_w / s.n_views;
+  int view = px / view_w;
+  int lx = px - view * view_w;
+  R jx = R(rng.next_1d()), jy = R(rng.next_1d());
+  R fx = (R(lx) + jx) / R(view_w), fy = (R(py) + jy) / R(s.film_h);
+  const float *v = s.views + 13 * view;
+  R aspect = R(view_w) / R(s.film_h);
+  R cx = (R(1) - R(2) * fx) * R(v[12]);
+  R cy = (R(1) - R(2) * fy) * R(v[12]) / aspect;
+  V<R> left(v[3], v[4], v[5]), up(v[6], v[7], v[8]), fwd(v[9], v[10], v[11]);
+  Ray<R> r;
+  r.o = V<R>(v[0], v[1], v[2]);
+  r.d = normalize(left * cx + up * cy + fwd);
+  r.maxt = std::numeric_limits<R>::infinity();
+  return r;
+}
+
+// Path-replay adjoint of the NEW state for one density parameter: primal pass
+// for L, replay with the same random numbers; dimage/dL = 1/spp (box filter).
+template <typename R>
+static void adjoint(const OracleScene &sc, const OracleIntegrator &I, uint32_t seed, int spp, const double *grad_img,
+                    int param, double *grad_out, int n_threads) {
+  Scene<R> S(sc);
+  const size_t n = size_t(sc.param_dims[4 * param]) * sc.param_dims[4 * param + 1] * sc.param_dims[4 * param + 2];
+  const int64_t n_pix = int64_t(sc.film_w) * sc.film_h;
+  std::atomic<int64_t> next(0);
+  n_threads = std::max(1, n_threads);
+  std::vector<std::vector<double>> bufs(n_threads, std::vector<double>(n, 0.0));
+  auto work = [&](int tid) {
+    while (true) {
+      int64_t pix = next.fetch_add(1);
+      if (pix >= n_pix) break;
+      int px = int(pix % sc.film_w), py = int(pix / sc.film_w);
+      V<R> dL(R(grad_img[3 * pix] / spp), R(grad_img[3 * pix + 1] / spp), R(grad_img[3 * pix + 2] / spp));
+      if (!any_nz(dL)) continue;
+      for (int s = 0; s < spp; ++s) {
+        Pcg32 rng;
+        uint32_t lane = uint32_t(pix * spp + s);
+        rng.seed(tea64(seed, lane), tea64(lane, seed));
+        Ray<R> ray = camera_ray<R>(sc, px, py, rng);
+        const Pcg32 rng0 = rng;
+        V<R> L[2];
+        volpath_twostates(S, I, rng, ray, L);
+        if (!any_nz(L[1])) continue;
+        Adj<R> a;
+        a.dL = dL;
+        a.Lrem = L[1];
+        a.param = param;
+        a.g = &bufs[tid];
+        rng = rng0;
+        V<R> L2[2];
+        volpath_twostates(S, I, rng, ray, L2, &a);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int i = 1; i < n_threads; ++i) th.emplace_back(work, i);
+  work(0);
+  for (auto &t : th) t.join();
+  for (size_t i = 0; i < n; ++i) {
+    double acc = 0;
+    for (int t = 0; t < n_threads; ++t) acc += bufs[t][i];
+    grad_out[i] = acc;
+  }
+}
+
+// Per pixel: mean and (optionally) the per-sample variance of L for both states.
+template <typename R>
+static void render(const OracleScene &sc, const OracleIntegrator &I, uint32_t seed, int spp, double *img_old,
+                   double *img_new, double *var_old, double *var_new, int n_threads, int64_t pixel_begin,
+                   int64_t pixel_end) {
+  Scene<R> S(sc);
+  std::atomic<int64_t> next(pixel_begin);
+  auto work = [&]() {
+    while (true) {
+      int64_t pix = next.fetch_add(1);
+      if (pix >= pixel_end) break;
+      int px = int(pix % sc.film_w), py = int(pix / sc.film_w);
+      double s1[2][3] = {{0, 0, 0}, {0, 0, 0}}, s2[2][3] = {{0, 0, 0}, {0, 0, 0}};
+      R acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+      for (int s = 0; s < spp; ++s) {
+        Pcg32 rng;
+        uint32_t lane = uint32_t(pix * spp + s);
+        rng.seed(tea64(seed, lane), tea64(lane, seed));
+        Ray<R> ray = camera_ray<R>(sc, px, py, rng);
+        V<R> L[2];
+        if (I.type == 1)
+          volpath_twostates(S, I, rng, ray, L);
+        else
+          path_twostates(S, I, rng, ray, L);
+        for (int k = 0; k < 2; ++k)
+          for (int c = 0; c < 3; ++c) {
+            acc[k][c] += L[k][c];
+            s1[k][c] += double(L[k][c]);
+            s2[k][c] += double(L[k][c]) * double(L[k][c]);
+          }
+      }
+      for (int c = 0; c < 3; ++c) {
+        img_old[3 * pix + c] = double(acc[0][c] / R(spp));
+        img_new[3 * pix + c] = double(acc[1][c] / R(spp));
+        if (var_old) var_old[3 * pix + c] = std::max(0.0, s2[0][c] / spp - (s1[0][c] / spp) * (s1[0][c] / spp));
+        if (var_new) var_new[3 * pix + c] = std::max(0.0, s2[1][c] / spp - (s1[1][c] / spp) * (s1[1][c] / spp));
+      }
+    }
+  };
+  n_threads = std::max(1, n_threads);
+  std::vector<std::thread> th;
+  for (int i = 1; i < n_threads; ++i) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+}
+
+}  // namespace oracle
+
+extern "C" {
+
+// Renders both states of sensor film pixels [pixel_begin, pixel_end)
+// (pixel_end <= 0: all). precision 32 or 64. Outputs are film-sized
+// (film_h, film_w, 3) double arrays; var_* may be NULL.
+int oracle_render(const OracleScene *s, const OracleIntegrator *I, uint32_t seed, int spp, int precision,
+                  double *img_old, double *img_new, double *var_old, double *var_new, int n_threads,
+                  int64_t pixel_begin, int64_t pixel_end) {
+  if (!s || !I || spp <= 0 || !img_old || !img_new) return -1;
+  int64_t n_pix = int64_t(s->film_w) * s->film_h;
+  if (pixel_end <= 0 || pixel_end > n_pix) pixel_end = n_pix;
+  if (precision == 64)
+    oracle::render<double>(*s, *I, seed, spp, img_old, img_new, var_old, var_new, n_threads, pixel_begin, pixel_end);
+  else
+    oracle::render<float>(*s, *I, seed, spp, img_old, img_new, var_old, var_new, n_threads, pixel_begin, pixel_end);
+  return 0;
+}
+
+// Volume adjoint (NEW state == param_new) into grad_out (grid-sized doubles).
+int oracle_adjoint(const OracleScene *s, const OracleIntegrator *I, uint32_t seed, int spp, int precision,
+                   const double *grad_img, int param, double *grad_out, int n_threads) {
+  if (!s || !I || spp <= 0 || !grad_img || !grad_out || param < 0 || param >= s->n_params || I->type != 1) return -1;
+  if (precision == 64)
+    oracle::adjoint<double>(*s, *I, seed, spp, grad_img, param, grad_out, n_threads);
+  else
+    oracle::adjoint<float>(*s, *I, seed, spp, grad_img, param, grad_out, n_threads);
+  return 0;
+}
+
+// PCG32 output stream for known-answer tests.
+int oracle_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t *out) {
+  oracle::Pcg32 r;
+  r.seed(initstate, initseq);
+  for (int i = 0; i < n; ++i) out[i] = r.next_u32();
+  return 0;
+}
+
+// TEA with an explicit key and round count.
+int oracle_tea(uint32_t v0, uint32_t v1, int rounds, const uint32_t *key, uint32_t *out) {
+  oracle::tea(v0, v1, rounds, key);
+  out[0] = v0;
+  out[1] = v1;
+  return 0;
+}
+}

@@ -1,0 +1,198 @@
+"""ctypes front end of the CPU render oracle (oracle/render_ref.cpp).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+
+``render(desc, integrator, seed, spp, params_old, params_new, precision)``
+renders both states of a SceneDescription (recursive_control_variates_amd.
+scene_desc) on the host; a plain render is the NEW image with
+``params_old == params_new``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "librcv_oracle.so")
+_lib = None
+
+
+class OracleScene(C.Structure):
+    _fields_ = [
+        ("n_tris", C.c_int), ("tri_v", C.c_void_p), ("tri_uv", C.c_void_p), ("tri_shape", C.c_void_p),
+        ("n_spheres", C.c_int), ("spheres", C.c_void_p), ("sphere_shape", C.c_void_p),
+        ("n_shapes", C.c_int), ("shapes", C.c_void_p),
+        ("n_bsdfs", C.c_int), ("bsdf_type", C.c_void_p), ("bsdf_tex", C.c_void_p), ("bsdf_rgb", C.c_void_p),
+        ("n_emitters", C.c_int), ("em_type", C.c_void_p), ("em_shape", C.c_void_p), ("em_rgb", C.c_void_p),
+        ("n_media", C.c_int), ("med_sigma", C.c_void_p), ("med_albedo", C.c_void_p),
+        ("med_albedo_rgb", C.c_void_p), ("med_scale", C.c_void_p), ("med_lo", C.c_void_p),
+        ("med_hi", C.c_void_p), ("med_sample_emitters", C.c_void_p),
+        ("n_params", C.c_int), ("param_dims", C.c_void_p), ("param_old", C.c_void_p), ("param_new", C.c_void_p),
+        ("film_w", C.c_int), ("film_h", C.c_int), ("n_views", C.c_int), ("views", C.c_void_p),
+    ]
+
+
+class OracleIntegrator(C.Structure):
+    _fields_ = [("type", C.c_int), ("max_depth", C.c_int), ("rr_depth", C.c_int), ("hide_emitters", C.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.oracle_render.restype = C.c_int
+        _lib.oracle_render.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleIntegrator), C.c_uint32, C.c_int,
+                                       C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_int64, C.c_int64]
+        _lib.oracle_adjoint.restype = C.c_int
+        _lib.oracle_adjoint.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleIntegrator), C.c_uint32, C.c_int,
+                                        C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        _lib.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
+        _lib.oracle_tea.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]
+    return _lib
+
+
+def _view_block(sensor):
+    out = []
+    for v in sensor.views:
+        m = np.asarray(v.to_world, np.float64)
+        left, up, fwd, org = m[:3, 0], m[:3, 1], m[:3, 2], m[:3, 3]
+        tan_half = np.tan(np.radians(v.fov) * 0.5)
+        out.append(np.concatenate([org, left, up, fwd, [tan_half]]))
+    return np.asarray(out, np.float32)
+
+
+class _Keep:
+    """Holds numpy buffers alive while the C struct points at them."""
+
+    def __init__(self):
+        self.bufs = []
+
+    def arr(self, a, dtype):
+        a = np.ascontiguousarray(np.asarray(a, dtype))
+        if a.size == 0:
+            a = np.zeros(1, dtype)
+        self.bufs.append(a)
+        return a.ctypes.data
+
+
+def make_scene(desc, params_old, params_new, sensor=0):
+    keep = _Keep()
+    s = OracleScene()
+    tv, tu, ts = desc.triangles()
+    sp, ss = desc.spheres()
+    s.n_tris = len(ts)
+    s.tri_v = keep.arr(tv, np.float32)
+    s.tri_uv = keep.arr(tu, np.float32)
+    s.tri_shape = keep.arr(ts, np.int32)
+    s.n_spheres = len(ss)
+    s.spheres = keep.arr(sp, np.float32)
+    s.sphere_shape = keep.arr(ss, np.int32)
+    s.n_shapes = len(desc.shapes)
+    s.shapes = keep.arr([[sh.bsdf, sh.emitter, sh.interior, sh.exterior] for sh in desc.shapes], np.int32)
+    s.n_bsdfs = len(desc.bsdfs)
+    s.bsdf_type = keep.arr([b.type for b in desc.bsdfs], np.int32)
+    s.bsdf_tex = keep.arr([b.texture_param for b in desc.bsdfs], np.int32)
+    s.bsdf_rgb = keep.arr([b.rgb for b in desc.bsdfs], np.float32)
+    s.n_emitters = len(desc.emitters)
+    s.em_type = keep.arr([e.type for e in desc.emitters], np.int32)
+    em_shape = []
+    for ei, e in enumerate(desc.emitters):
+        owner = [i for i, sh in enumerate(desc.shapes) if sh.emitter == ei]
+        em_shape.append(owner[0] if owner else -1)
+    s.em_shape = keep.arr(em_shape, np.int32)
+    s.em_rgb = keep.arr([e.rgb for e in desc.emitters], np.float32)
+    s.n_media = len(desc.media)
+    s.med_sigma = keep.arr([m.sigma_param for m in desc.media], np.int32)
+    s.med_albedo = keep.arr([m.albedo_param for m in desc.media], np.int32)
+    s.med_albedo_rgb = keep.arr([m.albedo_rgb for m in desc.media], np.float32)
+    s.med_scale = keep.arr([m.scale for m in desc.media], np.float32)
+    s.med_lo = keep.arr([m.bbox_min for m in desc.media], np.float32)
+    s.med_hi = keep.arr([m.bbox_max for m in desc.media], np.float32)
+    s.med_sample_emitters = keep.arr([int(m.sample_emitters) for m in desc.media], np.int32)
+    s.n_params = len(desc.params)
+    dims = []
+    for p in desc.params:
+        if len(p.shape) == 3:
+            dims.append([p.shape[1], p.shape[0], 1, p.shape[2]])
+        else:
+            dims.append([p.shape[2], p.shape[1], p.shape[0], p.shape[3]])
+    s.param_dims = keep.arr(dims, np.int32)
+    po = [keep.arr(a, np.float32) for a in params_old]
+    pn = [keep.arr(a, np.float32) for a in params_new]
+    s.param_old = keep.arr(po or [0], np.uint64)
+    s.param_new = keep.arr(pn or [0], np.uint64)
+    sen = desc.sensors[sensor]
+    s.film_w, s.film_h, s.n_views = sen.film_width, sen.film_height, len(sen.views)
+    s.views = keep.arr(_view_block(sen), np.float32)
+    return s, keep
+
+
+def render(desc, integrator_type, max_depth, rr_depth, seed, spp, params_old=None, params_new=None,
+           precision=64, sensor=0, n_threads=None, with_variance=False, hide_emitters=False,
+           pixel_range=None):
+    """Returns (img_old, img_new[, var_old, var_new]) as (H, W, 3) float64."""
+    if params_new is None:
+        params_new = [p.init for p in desc.params]
+    if params_old is None:
+        params_old = params_new
+    s, keep = make_scene(desc, params_old, params_new, sensor)
+    I = OracleIntegrator(int(integrator_type), int(max_depth), int(rr_depth), int(hide_emitters))
+    sen = desc.sensors[sensor]
+    shape = (sen.film_height, sen.film_width, 3)
+    io = np.zeros(shape, np.float64)
+    inew = np.zeros(shape, np.float64)
+    vo = np.zeros(shape, np.float64) if with_variance else None
+    vn = np.zeros(shape, np.float64) if with_variance else None
+    nt = n_threads or min(16, os.cpu_count() or 1)
+    b, e = pixel_range if pixel_range else (0, 0)
+    rc = lib().oracle_render(C.byref(s), C.byref(I), C.c_uint32(seed & 0xFFFFFFFF), spp, precision,
+                             io.ctypes.data, inew.ctypes.data, vo.ctypes.data if vo is not None else None,
+                             vn.ctypes.data if vn is not None else None, nt, b, e)
+    if rc != 0:
+        raise RuntimeError("oracle_render failed")
+    del keep
+    if with_variance:
+        return io, inew, vo, vn
+    return io, inew
+
+
+def adjoint(desc, max_depth, rr_depth, seed, spp, grad_img, param=0, params=None, precision=64,
+            n_threads=None, hide_emitters=False):
+    """Volume path-replay adjoint of the NEW state: d<grad_img, img>/d param."""
+    if params is None:
+        params = [p.init for p in desc.params]
+    s, keep = make_scene(desc, params, params, 0)
+    I = OracleIntegrator(1, int(max_depth), int(rr_depth), int(hide_emitters))
+    gi = np.ascontiguousarray(grad_img, np.float64)
+    out = np.zeros(desc.params[param].shape[:3], np.float64)
+    nt = n_threads or min(16, os.cpu_count() or 1)
+    rc = lib().oracle_adjoint(C.byref(s), C.byref(I), C.c_uint32(seed & 0xFFFFFFFF), spp, precision,
+                              gi.ctypes.data, param, out.ctypes.data, nt)
+    if rc != 0:
+        raise RuntimeError("oracle_adjoint failed")
+    del keep
+    return out[..., None]
+
+
+def pcg32(initstate, initseq, n):
+    out = np.zeros(n, np.uint32)
+    lib().oracle_pcg32(C.c_uint64(initstate), C.c_uint64(initseq), n, out.ctypes.data)
+    return out
+
+
+def tea(v0, v1, rounds, key):
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(2, np.uint32)
+    lib().oracle_tea(C.c_uint32(v0), C.c_uint32(v1), rounds, k.ctypes.data, out.ctypes.data)
+    return int(out[0]), int(out[1])

@@ -1,0 +1,280 @@
+// Render entry points of librcv: plain, two-state and adjoint launches of the
+// surface (path.hpp) and volume (volpath.hpp) integrators.
+//
+// Launch shape: one thread traces `spt` camera samples of one pixel in
+// sequence (samples_per_thread); the spp/spt threads of a pixel sit in the
+// same wavefront and their sums are combined with cross-lane shuffles, so the
+// film needs no atomics and the pixel average (box filter develop, plugins/
+// twostatepath.py:168-177) is written once, coalesced. The random stream of a
+// camera sample depends only on (seed, pixel * spp + sample), so the launch
+// shape changes results only through the float summation order of a pixel.
+// The BVH traversal stack of each thread lives in LDS (kStackSize ints,
+// strided by the block size).
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "path.hpp"
+#include "render_internal.hpp"
+#include "volpath.hpp"
+
+namespace rcv {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline void flush_counters(const SceneView &S, LocalCounters &cnt) {
+  if (!S.counters) return;
+#pragma unroll
+  for (int i = 0; i < kNumCounters; ++i) {
+    unsigned v = cnt.c[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&S.counters[i], (unsigned long long)v);
+  }
+}
+
+template <int NS, int INTEG>
+__global__ __launch_bounds__(kBlock) void render_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed,
+                                                        int spp, int spt, int chunk_log2, float *img_old,
+                                                        float *img_new) {
+  __shared__ int lds_stack[kStackSize * kBlock];
+  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  const SensorInfo &sen = S.sensors[sensor];
+  const int film_w = sen.film_w;
+  const int64_t n_pix = int64_t(film_w) * sen.film_h;
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t pixel = tid >> chunk_log2;
+  const int chunk = int(tid & ((1 << chunk_log2) - 1));
+  V3 acc[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) acc[k] = v3(0.f);
+  LocalCounters cnt;
+  cnt.zero();
+  if (pixel < n_pix) {
+    const int px = int(pixel % film_w), py = int(pixel / film_w);
+    for (int j = 0; j < spt; ++j) {
+      const int s = chunk * spt + j;
+      Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+      const Ray ray = camera_ray(sen, px, py, rng);
+      V3 L[NS];
+      if (INTEG == RCV_INTEGRATOR_VOLPATH)
+        volpath_sample<NS, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+      else
+        path_sample<NS, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+#pragma unroll
+      for (int k = 0; k < NS; ++k) acc[k] += L[k];
+    }
+  }
+  for (int off = 1; off < (1 << chunk_log2); off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      acc[k].x += __shfl_xor(acc[k].x, off, 64);
+      acc[k].y += __shfl_xor(acc[k].y, off, 64);
+      acc[k].z += __shfl_xor(acc[k].z, off, 64);
+    }
+  }
+  if (pixel < n_pix && chunk == 0) {
+    const float inv = float(spp);
+    float *o0 = (NS == 2 ? img_old : img_new) + 3 * pixel;
+    o0[0] = acc[0].x / inv;
+    o0[1] = acc[0].y / inv;
+    o0[2] = acc[0].z / inv;
+    if (NS == 2) {
+      float *o1 = img_new + 3 * pixel;
+      o1[0] = acc[NS - 1].x / inv;
+      o1[1] = acc[NS - 1].y / inv;
+      o1[2] = acc[NS - 1].z / inv;
+    }
+  }
+  flush_counters(S, cnt);
+}
+
+// Path-replay adjoint (NEW state): primal pass for L, then a replay with the
+// same random numbers that scatters d(log weight)/d(theta) times the radiance
+// still to come into the parameter gradients. dimage/dL = 1/spp (box filter).
+template <int INTEG>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed,
+                                                         int spp, int spt, int chunk_log2,
+                                                         const float *__restrict__ grad_img, GradTargets G) {
+  __shared__ int lds_stack[kStackSize * kBlock];
+  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  const SensorInfo &sen = S.sensors[sensor];
+  const int film_w = sen.film_w;
+  const int64_t n_pix = int64_t(film_w) * sen.film_h;
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t pixel = tid >> chunk_log2;
+  const int chunk = int(tid & ((1 << chunk_log2) - 1));
+  LocalCounters cnt;
+  cnt.zero();
+  if (pixel < n_pix) {
+    const int px = int(pixel % film_w), py = int(pixel / film_w);
+    const float inv = float(spp);
+    const V3 dL = v3(grad_img[3 * pixel] / inv, grad_img[3 * pixel + 1] / inv, grad_img[3 * pixel + 2] / inv);
+    if (any_nonzero(dL)) {
+      for (int j = 0; j < spt; ++j) {
+        const int s = chunk * spt + j;
+        Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+        const Ray ray = camera_ray(sen, px, py, rng);
+        const Pcg32 rng0 = rng;
+        V3 L[1];
+        if (INTEG == RCV_INTEGRATOR_VOLPATH)
+          volpath_sample<1, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+        else
+          path_sample<1, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+        if (!any_nonzero(L[0])) continue;
+        AdjState adj{dL, L[0]};
+        rng = rng0;
+        V3 L2[1];
+        if (INTEG == RCV_INTEGRATOR_VOLPATH)
+          volpath_sample<1, true>(S, I, rng, ray, L2, stack, cnt, &adj, &G);
+        else
+          path_sample<1, true>(S, I, rng, ray, L2, stack, cnt, &adj, &G);
+      }
+    }
+  }
+  flush_counters(S, cnt);
+}
+
+__global__ void zero_counters_kernel(unsigned long long *c) {
+  if (threadIdx.x < kNumCounters) c[threadIdx.x] = 0ull;
+}
+
+int pow2_part(int v) { return v & -v; }
+
+}  // namespace
+
+int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2) {
+  int chunks;
+  const int cap = std::min(64, pow2_part(spp));
+  if (d && d->samples_per_thread > 0) {
+    if (spp % d->samples_per_thread != 0)
+      return fail(RCV_E_INVALID, "samples_per_thread %d must divide spp %d", d->samples_per_thread, spp);
+    chunks = spp / d->samples_per_thread;
+    if (chunks > 64 || (chunks & (chunks - 1)) != 0)
+      return fail(RCV_E_INVALID, "spp / samples_per_thread = %d must be a power of two <= 64", chunks);
+  } else {
+    chunks = 1;
+    while (chunks < cap && n_pix * chunks < (int64_t(1) << 19)) chunks <<= 1;
+  }
+  spt = spp / chunks;
+  chunk_log2 = 0;
+  while ((1 << chunk_log2) < chunks) ++chunk_log2;
+  return RCV_OK;
+}
+
+static int check_states(const Scene *s, int ns, const char *fn) {
+  auto need = [&](int param, const char *what) -> int {
+    if (param < 0) return RCV_OK;
+    const Param &p = s->params[param];
+    if (!p.data[RCV_STATE_NEW]) return fail(RCV_E_STATE, "%s: %s parameter %d has no NEW state bound", fn, what, param);
+    if (ns == 2 && !p.data[RCV_STATE_OLD])
+      return fail(RCV_E_STATE, "%s: %s parameter %d has no OLD state bound", fn, what, param);
+    return RCV_OK;
+  };
+  for (auto &b : s->bsdfs)
+    if (int rc = need(b.texture_param, "texture")) return rc;
+  for (auto &m : s->media) {
+    if (int rc = need(m.sigma_param, "sigma_t")) return rc;
+    if (int rc = need(m.albedo_param, "albedo")) return rc;
+  }
+  return RCV_OK;
+}
+
+int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint32_t seed, int spp,
+                  float *img_old, float *img_new, int ns, hipStream_t stream, const char *fn) {
+  RCV_REQUIRE(s && s->committed, "%s: scene not committed", fn);
+  RCV_REQUIRE(d, "%s: null integrator", fn);
+  RCV_REQUIRE(d->type == RCV_INTEGRATOR_PATH || d->type == RCV_INTEGRATOR_VOLPATH, "%s: bad integrator type", fn);
+  RCV_REQUIRE(sensor >= 0 && sensor < int(s->sensors.size()), "%s: bad sensor %d", fn, sensor);
+  RCV_REQUIRE(spp > 0, "%s: spp must be positive", fn);
+  RCV_REQUIRE(img_new && (ns == 1 || img_old), "%s: null image", fn);
+  if (int rc = check_states(s, ns, fn)) return rc;
+  const SensorInfo &sen = s->sensors[sensor];
+  const int64_t n_pix = int64_t(sen.film_w) * sen.film_h;
+  RCV_REQUIRE(n_pix * spp < (int64_t(1) << 32), "%s: pixels * spp must stay below 2^32", fn);
+  int spt, cl2;
+  if (int rc = plan_launch(d, n_pix, spp, spt, cl2)) return rc;
+  const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters};
+  const int64_t threads = n_pix << cl2;
+  const unsigned grid = ceil_div(threads, kBlock);
+  if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
+  if (d->type == RCV_INTEGRATOR_VOLPATH) {
+    if (ns == 2)
+      hipLaunchKernelGGL((render_kernel<2, RCV_INTEGRATOR_VOLPATH>), dim3(grid), dim3(kBlock), 0, stream, s->view,
+                         I, sensor, seed, spp, spt, cl2, img_old, img_new);
+    else
+      hipLaunchKernelGGL((render_kernel<1, RCV_INTEGRATOR_VOLPATH>), dim3(grid), dim3(kBlock), 0, stream, s->view,
+                         I, sensor, seed, spp, spt, cl2, img_old, img_new);
+  } else {
+    if (ns == 2)
+      hipLaunchKernelGGL((render_kernel<2, RCV_INTEGRATOR_PATH>), dim3(grid), dim3(kBlock), 0, stream, s->view, I,
+                         sensor, seed, spp, spt, cl2, img_old, img_new);
+    else
+      hipLaunchKernelGGL((render_kernel<1, RCV_INTEGRATOR_PATH>), dim3(grid), dim3(kBlock), 0, stream, s->view, I,
+                         sensor, seed, spp, spt, cl2, img_old, img_new);
+  }
+  return check_launch(fn);
+}
+
+int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uint32_t seed, int spp,
+                   const float *grad_img, int n_params, const int32_t *param_ids, float *const *grad_ptrs,
+                   hipStream_t stream) {
+  const char *fn = "rcv_render_backward";
+  RCV_REQUIRE(s && s->committed, "%s: scene not committed", fn);
+  RCV_REQUIRE(d, "%s: null integrator", fn);
+  RCV_REQUIRE(sensor >= 0 && sensor < int(s->sensors.size()), "%s: bad sensor %d", fn, sensor);
+  RCV_REQUIRE(spp > 0 && grad_img, "%s: bad spp or null grad_img", fn);
+  RCV_REQUIRE(n_params >= 0 && (n_params == 0 || (param_ids && grad_ptrs)), "%s: bad gradient targets", fn);
+  if (int rc = check_states(s, 1, fn)) return rc;
+  GradTargets G;
+  for (int i = 0; i < kMaxParams; ++i) G.g[i] = nullptr;
+  for (int i = 0; i < n_params; ++i) {
+    RCV_REQUIRE(param_ids[i] >= 0 && param_ids[i] < int(s->params.size()), "%s: bad param id %d", fn, param_ids[i]);
+    RCV_REQUIRE(grad_ptrs[i], "%s: null gradient buffer for param %d", fn, param_ids[i]);
+    G.g[param_ids[i]] = grad_ptrs[i];
+  }
+  const SensorInfo &sen = s->sensors[sensor];
+  const int64_t n_pix = int64_t(sen.film_w) * sen.film_h;
+  RCV_REQUIRE(n_pix * spp < (int64_t(1) << 32), "%s: pixels * spp must stay below 2^32", fn);
+  int spt, cl2;
+  if (int rc = plan_launch(d, n_pix, spp, spt, cl2)) return rc;
+  const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters};
+  const unsigned grid = ceil_div(n_pix << cl2, kBlock);
+  if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
+  if (d->type == RCV_INTEGRATOR_VOLPATH)
+    hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH>), dim3(grid), dim3(kBlock), 0, stream, s->view, I,
+                       sensor, seed, spp, spt, cl2, grad_img, G);
+  else
+    hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_PATH>), dim3(grid), dim3(kBlock), 0, stream, s->view, I,
+                       sensor, seed, spp, spt, cl2, grad_img, G);
+  return check_launch(fn);
+}
+
+int Scene::build_supergrid(int, int, hipStream_t) {
+  return fail(RCV_E_INVALID, "majorant supergrid: not supported by this build");
+}
+
+}  // namespace rcv
+
+using namespace rcv;
+
+extern "C" int rcv_render(const rcv_scene *scene, const rcv_integrator_desc *integrator, int32_t sensor,
+                          uint32_t seed, int32_t spp, float *img, void *stream) {
+  return launch_render(static_cast<const Scene *>(scene), integrator, sensor, seed, spp, nullptr, img, 1,
+                       as_stream(stream), "rcv_render");
+}
+
+extern "C" int rcv_render_twostates(const rcv_scene *scene, const rcv_integrator_desc *integrator,
+                                    int32_t sensor, uint32_t seed, int32_t spp, float *img_old, float *img_new,
+                                    void *stream) {
+  return launch_render(static_cast<const Scene *>(scene), integrator, sensor, seed, spp, img_old, img_new, 2,
+                       as_stream(stream), "rcv_render_twostates");
+}
+
+extern "C" int rcv_render_backward(const rcv_scene *scene, const rcv_integrator_desc *integrator, int32_t sensor,
+                                   uint32_t seed, int32_t spp, const float *grad_img, int32_t n_params,
+                                   const int32_t *param_ids, float *const *grad_ptrs, void *stream) {
+  return launch_adjoint(static_cast<const Scene *>(scene), integrator, sensor, seed, spp, grad_img, n_params,
+                        param_ids, grad_ptrs, as_stream(stream));
+}

@@ -1,0 +1,62 @@
+// Host-side scene object behind the opaque rcv_scene handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "scene_types.hpp"
+
+namespace rcv {
+
+struct Param {
+  int kind = 0;
+  int nx = 0, ny = 0, nz = 0, ch = 0;
+  const float *data[2] = {nullptr, nullptr};
+};
+
+struct Scene {
+  // collected before commit
+  std::vector<Param> params;
+  std::vector<BsdfInfo> bsdfs;
+  std::vector<MediumInfo> media;
+  std::vector<EmitterInfo> emitters;
+  std::vector<ShapeInfo> shapes;
+  std::vector<SensorInfo> sensors;
+  std::vector<float> tri_v;     // 9 floats per triangle
+  std::vector<float> tri_uv;    // 6 floats per triangle
+  std::vector<int32_t> tri_shape;
+  std::vector<float> sph;       // center xyz, radius
+  std::vector<int32_t> sph_shape;
+
+  // after commit
+  bool committed = false;
+  void *dev_block = nullptr;
+  uint32_t *majorant_keys = nullptr;  // [param][state], order-preserving float keys
+  float *supergrids = nullptr;
+  unsigned long long *counters = nullptr;
+  int n_nodes = 0;
+  float bs_center[3] = {0, 0, 0};
+  float bs_radius = 0;
+  SceneView view{};
+
+  Scene() = default;
+  Scene(const Scene &) = delete;
+  Scene &operator=(const Scene &) = delete;
+  ~Scene();
+
+  int upload(hipStream_t stream);
+  int reduce_param(int param, int state, hipStream_t stream);
+  int build_supergrid(int medium, int state, hipStream_t stream);
+};
+
+// Decodes an order-preserving key written by the majorant reduction.
+__host__ __device__ inline float key_to_float(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __builtin_bit_cast(float, u);
+}
+
+}  // namespace rcv
+
+// The opaque C handle is the scene object itself.
+struct rcv_scene : public rcv::Scene {};

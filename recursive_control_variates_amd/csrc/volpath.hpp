@@ -1,0 +1,557 @@
+// Two-state volumetric path tracer (K3), its single-state form (the plain
+// prbvolpath render used by the baseline method) and the path-replay adjoint
+// (K5) on the NEW state.
+//
+// Follows plugins/twostatevolpath.py:141-479 and plugins/twostatemedium.py:
+// 15-107 lane for lane: random numbers are drawn in the same order and only by
+// the lanes the reference draws them for, weights are formed with the same
+// expressions, and the loop-mask semantics (e.g. emitter hits are still
+// gathered in the iteration where depth reaches max_depth, :253,284-308) are
+// kept. NS = 2 carries (old, new) throughputs along one path; NS = 1 uses the
+// NEW state only and reproduces the NS = 2 result of a scene whose two states
+// are equal bit for bit (2a/(a+a) == 1 and (x+x)*0.5 == x in IEEE arithmetic).
+//
+// Extinction is grey (1-channel sigma_t grid times `scale`, as in the
+// reference's experiments where spectral extinction is disabled), so the
+// colour channel the reference samples for free-flight distances
+// (:171-173) does not change any result; its random number is still drawn.
+#pragma once
+
+#include "scene_device.hpp"
+
+namespace rcv {
+
+struct IntegArgs {
+  int max_depth;
+  int rr_depth;
+  int hide_emitters;
+};
+
+struct GradTargets {
+  float *g[kMaxParams];
+};
+
+struct LocalCounters {
+  unsigned c[kNumCounters];
+  __device__ void zero() {
+#pragma unroll
+    for (int i = 0; i < kNumCounters; ++i) c[i] = 0;
+  }
+};
+
+template <int NS>
+__device__ constexpr int state_index(int k) {
+  return NS == 2 ? k : RCV_STATE_NEW;
+}
+
+template <int NS>
+struct MediumEv {
+  bool valid;
+  float t, mint, maj;
+  V3 p;
+  float st[NS];  // sigma_t
+  float sn[NS];  // sigma_n = majorant - sigma_t
+  V3 ss[NS];     // sigma_s = albedo * sigma_t
+  V3 alb;        // NEW-state albedo (adjoint)
+  GridFoot foot;
+};
+
+__device__ inline float majorant_of(const SceneView &S, int param, int state) {
+  return key_to_float(reinterpret_cast<const uint32_t *>(S.majorants)[2 * param + state]);
+}
+
+// Ray/box slab test returning the parametric interval (may start behind the
+// origin).
+__device__ inline bool ray_aabb(const Ray &r, const float lo[3], const float hi[3], float &t0, float &t1) {
+  t0 = -kInf;
+  t1 = kInf;
+  const float o[3] = {r.o.x, r.o.y, r.o.z};
+  const float d[3] = {r.d.x, r.d.y, r.d.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float inv = 1.0f / d[k];
+    float ta = (lo[k] - o[k]) * inv;
+    float tb = (hi[k] - o[k]) * inv;
+    if (ta > tb) {
+      const float tmp = ta;
+      ta = tb;
+      tb = tmp;
+    }
+    t0 = ta > t0 ? ta : t0;
+    t1 = tb < t1 ? tb : t1;
+  }
+  return t0 <= t1;
+}
+
+// TwoStateMedium.sample_interaction_twostates (plugins/twostatemedium.py:
+// 15-107) with one global majorant max(maj_old, maj_new) (:71-78).
+template <int NS, bool FOOT>
+__device__ inline void sample_medium(const SceneView &S, int mid, const Ray &ray, float u, MediumEv<NS> &ev) {
+  const MediumInfo &M = S.media[mid];
+  float t0, t1;
+  bool ok = ray_aabb(ray, M.lo, M.hi, t0, t1);
+  ok = ok && (t0 < kInf || t1 < kInf);
+  const float mint = ok ? fmaxf(0.0f, t0) : 0.0f;
+  const float maxt = ok ? fminf(ray.maxt, t1) : kInf;
+  const float scale = M.scale;
+  float maj = majorant_of(S, M.sigma_param, RCV_STATE_NEW);
+  if (NS == 2) maj = fmaxf(maj, majorant_of(S, M.sigma_param, RCV_STATE_OLD));
+  maj *= scale;
+  // A medium whose majorant is not a positive finite number has no
+  // collisions (t = inf for a zero majorant, as in the reference); this also
+  // keeps every sampled step strictly forward.
+  ok = ok && maj > 0.0f && maj < kInf;
+  const float tau = -logf(1.0f - u);
+  const float t = mint + tau / maj;
+  ev.valid = ok && (t <= maxt);
+  ev.mint = mint;
+  ev.maj = maj;
+  ev.t = ev.valid ? t : kInf;
+  ev.p = ray_at(ray, t);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    ev.st[k] = 0.0f;
+    ev.ss[k] = v3(0.f);
+  }
+  ev.alb = v3(0.f);
+  if (ev.valid) {
+    const ParamBind &P = S.params[M.sigma_param];
+    grid_footprint(P, M, ev.p, ev.foot);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const int s = state_index<NS>(k);
+      ev.st[k] = scale * grid_eval1(P.data[s], ev.foot);
+      V3 alb;
+      if (M.albedo_param >= 0) {
+        const ParamBind &A = S.params[M.albedo_param];
+        GridFoot fa;
+        grid_footprint(A, M, ev.p, fa);
+        alb = grid_eval3(A.data[s], fa);
+      } else {
+        alb = v3(M.albedo[0], M.albedo[1], M.albedo[2]);
+      }
+      if (k == NS - 1) ev.alb = alb;
+      ev.ss[k] = alb * ev.st[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) ev.sn[k] = maj - ev.st[k];
+}
+
+__device__ inline void scatter_grid(float *g, const GridFoot &f, float val) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) atomicAdd(g + f.idx(c), val * f.w(c));
+}
+
+__device__ inline void scatter_grid3(float *g, const GridFoot &f, V3 val) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float w = f.w(c);
+    float *t = g + 3 * f.idx(c);
+    atomicAdd(t + 0, val.x * w);
+    atomicAdd(t + 1, val.y * w);
+    atomicAdd(t + 2, val.z * w);
+  }
+}
+
+__device__ inline bool null_bsdf(const SceneView &S, int shape) {
+  const int b = S.shapes[shape].bsdf;
+  return b >= 0 && S.bsdfs[b].type == RCV_BSDF_NULL;
+}
+
+// sample_emitter_twostates (plugins/twostatevolpath.py:387-479): emitter
+// direction without visibility, then ratio tracking of both transmittances.
+// GRAD = true replays the same random numbers and scatters
+// d(log T)/d(density) * gscale into the density gradient.
+template <int NS, bool GRAD>
+__device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, bool ref_surface, int medium,
+                                     Pcg32 &rng, V3 em[NS], DirSample &ds, const LdsStack &stack,
+                                     LocalCounters &cnt, float gscale, const GradTargets *G) {
+  const float u1 = rng.next_1d();
+  const float u2 = rng.next_1d();
+  const V3 ev = sample_emitter_direction(S, ref_p, u1, u2, ds);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) em[k] = v3(0.f);
+  if (ds.pdf == 0.0f) return;
+  Ray ray;
+  ray.o = ref_surface ? offset_point(ref_p, ref_n, ds.d) : ref_p;
+  ray.d = ds.d;
+  float total = 0.0f;
+  bool needs_int = true;
+  SurfInt si;
+  si.valid = false;
+  si.t = kInf;
+  float T[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) T[k] = 1.0f;
+  while (true) {
+    const float remaining = ds.dist * (1.0f - kShadowEps) - total;
+    ray.maxt = remaining;
+    if (!(remaining > 0.0f)) break;
+    if (needs_int) {
+      Hit h;
+      trace<false>(S, ray, h, stack);
+      cnt.c[kCntTraversals]++;
+      si = make_si(S, ray, h);
+      needs_int = false;
+    }
+    bool active_medium = medium >= 0;
+    bool active_surface = !active_medium;
+    MediumEv<NS> mev;
+    mev.valid = false;
+    mev.t = kInf;
+    if (active_medium) {
+      const float u = rng.next_1d();
+      sample_medium<NS, GRAD>(S, medium, ray, u, mev);
+      cnt.c[kCntNeeSteps]++;
+      if (si.t < mev.t) {
+        mev.valid = false;
+        mev.t = kInf;
+      }
+    }
+    const bool escaped = active_medium && !mev.valid;
+    active_medium = active_medium && mev.valid;
+    float trm[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) trm[k] = 1.0f;
+    if (active_medium) {
+      ray.o = mev.p;
+      si.t = si.t - mev.t;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) trm[k] = mev.maj != 0.0f ? mev.sn[k] / mev.maj : mev.sn[k];
+      if (GRAD) {
+        const MediumInfo &M = S.media[medium];
+        float *g = G->g[M.sigma_param];
+        if (g && mev.sn[NS - 1] != 0.0f) {
+          scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
+          cnt.c[kCntGradAtomics] += 8;
+        }
+      }
+    }
+    active_surface = (active_surface || escaped) && si.valid && !active_medium;
+    if (active_surface) {
+      const float b = null_bsdf(S, si.shape) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) trm[k] = trm[k] * b;
+    }
+    bool any_t = false;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      T[k] *= trm[k];
+      any_t = any_t || T[k] != 0.0f;
+    }
+    if (active_surface) {
+      ray = spawn_ray(si, ray.d);
+      needs_int = true;
+    }
+    const bool active = (active_medium || active_surface) && any_t;
+    if (!active) break;
+    total += active_medium ? mev.t : si.t;
+    if (active_surface) {
+      const ShapeInfo sh = S.shapes[si.shape];
+      if (sh.interior != sh.exterior) medium = dot(si.n, ray.d) > 0.0f ? sh.exterior : sh.interior;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) em[k] = ev * T[k];
+}
+
+struct AdjState {
+  V3 dL;    // dLoss/dL of this camera sample
+  V3 Lrem;  // radiance still to be gathered along the replayed path
+};
+
+// One camera sample. NS = 2: L[0] = old (H), L[1] = new (F). REPLAY (NS = 1):
+// second pass of path-replay backpropagation; the primal radiance has been
+// stored in adj->Lrem by the caller.
+template <int NS, bool REPLAY>
+__device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, Ray ray, V3 L[NS],
+                               const LdsStack &stack, LocalCounters &cnt, AdjState *adj,
+                               const GradTargets *G) {
+  V3 thr[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    L[k] = v3(0.f);
+    thr[k] = v3(1.f);
+  }
+  float eta = 1.0f;
+  int depth = 0;
+  bool specular_chain = true;
+  SurfInt si;
+  si.valid = false;
+  si.t = kInf;
+  si.shape = -1;
+  si.p = v3(0.f);
+  si.n = v3(0.f, 0.f, 1.f);
+  bool needs_int = true;
+  V3 last_p = v3(0.f);
+  float last_pdf = 1.0f;
+  int medium = -1;
+  const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
+  (void)rng.next_1d();  // colour channel for free-flight sampling (:171-173)
+  cnt.c[kCntPaths]++;
+
+  while (true) {
+    // ---- Russian roulette (:182-188)
+    bool nonzero = false;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) nonzero = nonzero || any_nonzero(thr[k]);
+    if (!nonzero) break;
+    const V3 tm = NS == 2 ? vmax(thr[NS - 1], thr[0]) : thr[0];
+    const float q = fminf(hmax(tm) * (eta * eta), 0.99f);
+    const bool perform_rr = depth > I.rr_depth;
+    const float urr = rng.next_1d();
+    if (perform_rr && !(urr < q)) break;
+    if (perform_rr) {
+      const float rq = 1.0f / q;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) thr[k] = thr[k] * rq;
+    }
+
+    bool active_medium = medium >= 0;
+    bool active_surface = !active_medium;
+
+    // ---- free flight with the combined majorant (:193-206)
+    MediumEv<NS> mev;
+    mev.valid = false;
+    mev.t = kInf;
+    if (active_medium) {
+      const float u = rng.next_1d();
+      sample_medium<NS, REPLAY>(S, medium, ray, u, mev);
+      cnt.c[kCntCollisions]++;
+      if (needs_int) {
+        Hit h;
+        trace<false>(S, ray, h, stack);
+        cnt.c[kCntTraversals]++;
+        si = make_si(S, ray, h);
+      }
+      needs_int = false;
+      if (si.t < mev.t) {
+        mev.valid = false;
+        mev.t = kInf;
+      }
+    }
+    const bool escaped = active_medium && !mev.valid;
+    active_medium = active_medium && mev.valid;
+
+    // ---- null / real decision with the averaged probability (:219-231)
+    bool act_null = false, act_scatter = false;
+    V3 w[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) w[k] = v3(1.f);
+    if (active_medium) {
+      const float maj = mev.maj;
+      float p;
+      if (NS == 2)
+        p = maj != 0.0f ? (mev.st[1] / maj + mev.st[0] / maj) * 0.5f : 0.0f;
+      else
+        p = maj != 0.0f ? mev.st[0] / maj : 0.0f;
+      const float un = rng.next_1d();
+      act_null = un >= p;
+      act_scatter = !act_null;
+      if (act_null) cnt.c[kCntNullEvents]++;
+      if (NS == 2 && act_null) {
+        const float den = mev.sn[1] + mev.sn[0];
+        w[1] = w[1] * (2.0f * mev.sn[1] / den);
+        w[0] = w[0] * (2.0f * mev.sn[0] / den);
+      }
+    }
+    if (act_scatter) {
+      depth += 1;
+      last_p = mev.p;
+    }
+    const bool alive = depth < max_depth;  // (:253)
+    act_scatter = act_scatter && alive;
+    if (act_null) {
+      ray.o = mev.p;
+      si.t = si.t - mev.t;
+    }
+    if (act_scatter) {
+      cnt.c[kCntRealScatter]++;
+      if (NS == 2) {
+        const float den = mev.st[1] + mev.st[0];
+        w[1] = w[1] * (2.0f * mev.ss[1] / den);
+        w[0] = w[0] * (2.0f * mev.ss[0] / den);
+      } else {
+        w[0] = w[0] * (mev.ss[0] / mev.st[0]);
+      }
+    }
+    if (active_medium) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) thr[k] *= w[k];
+    }
+    if (REPLAY && (act_null || act_scatter)) {
+      // d log(weight) / d theta at this collision times the radiance that
+      // the weight multiplies (everything gathered after this point).
+      const MediumInfo &M = S.media[medium];
+      float *gd = G->g[M.sigma_param];
+      const float lr = dot(adj->dL, adj->Lrem);
+      if (gd && lr != 0.0f) {
+        if (act_null) {
+          if (mev.sn[0] != 0.0f) scatter_grid(gd, mev.foot, -lr * M.scale / mev.sn[0]);
+        } else {
+          scatter_grid(gd, mev.foot, lr * M.scale / mev.st[0]);
+        }
+        cnt.c[kCntGradAtomics] += 8;
+      }
+      if (act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
+        const V3 dl = adj->dL * adj->Lrem;
+        const V3 ga = v3(mev.alb.x != 0.f ? dl.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? dl.y / mev.alb.y : 0.f,
+                         mev.alb.z != 0.f ? dl.z / mev.alb.z : 0.f);
+        GridFoot fa;
+        grid_footprint(S.params[M.albedo_param], M, mev.p, fa);
+        scatter_grid3(G->g[M.albedo_param], fa, ga);
+        cnt.c[kCntGradAtomics] += 24;
+      }
+    }
+
+    // ---- phase sampling (:268-281), isotropic
+    if (act_scatter) {
+      (void)rng.next_1d();
+      const float a = rng.next_1d();
+      const float b = rng.next_1d();
+      const V3 wo = square_to_uniform_sphere(a, b);
+      ray.o = mev.p;
+      ray.d = wo;
+      ray.maxt = kInf;
+      needs_int = true;
+      last_pdf = kInv4Pi;
+    }
+
+    // ---- surface interactions and emitter hits (:284-308)
+    active_surface = active_surface || escaped;
+    if (active_surface && needs_int) {
+      Hit h;
+      trace<false>(S, ray, h, stack);
+      cnt.c[kCntTraversals]++;
+      si = make_si(S, ray, h);
+    }
+    if (active_surface) {
+      const int em = si_emitter(S, si);
+      if (em >= 0 && !(depth == 0 && I.hide_emitters)) {
+        const bool count_direct = depth == 0 || specular_chain;
+        const DirSample ds = dir_sample_from_hit(S, si, last_p, ray.d);
+        const float em_pdf = pdf_emitter_direction(S, ds);
+        const V3 emitted = emitter_eval(S, em, si, ray.d);
+        const float mis = mis_weight(last_pdf, em_pdf);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          const V3 c = count_direct ? thr[k] * emitted : thr[k] * mis * emitted;
+          L[k] += c;
+          if (REPLAY) adj->Lrem -= c;
+        }
+      }
+    }
+    active_surface = active_surface && si.valid;
+    BsdfInfo bsdf;
+    bsdf.type = -1;
+    bool smooth = false;
+    Frame fr;
+    V3 wi = v3(0.f, 0.f, 1.f);
+    if (active_surface) {
+      const int b = S.shapes[si.shape].bsdf;
+      if (b >= 0) bsdf = S.bsdfs[b];
+      smooth = bsdf.type == RCV_BSDF_DIFFUSE;
+      fr = make_frame(si.n);
+      wi = fr.to_local(-ray.d);
+    }
+
+    // ---- emitter sampling (:315-344)
+    const bool active_e_surface = active_surface && smooth && (depth + 1 < max_depth);
+    const bool sample_emitters = act_scatter && S.media[medium].sample_emitters != 0;
+    specular_chain = specular_chain && !act_scatter;
+    specular_chain = specular_chain || (act_scatter && !sample_emitters);
+    const bool active_e_medium = act_scatter && sample_emitters;
+    if (active_e_surface || active_e_medium) {
+      const V3 ref_p = active_e_medium ? mev.p : si.p;
+      V3 em[NS];
+      DirSample ds;
+      const Pcg32 rng_nee = rng;
+      nee_twostates<NS, false>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G);
+      V3 nw[NS];
+      float nee_pdf;
+      if (active_e_surface) {
+        const V3 wo = fr.to_local(ds.d);
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+          nw[k] = diffuse_eval(diffuse_reflectance(S, bsdf, state_index<NS>(k), si.u, si.v), wi, wo);
+        nee_pdf = diffuse_pdf(wi, wo);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) nw[k] = v3(kInv4Pi);
+        nee_pdf = kInv4Pi;
+      }
+      const float mis_em = mis_weight(ds.pdf, nee_pdf);
+      V3 c[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        c[k] = thr[k] * nw[k] * mis_em * em[k];
+        L[k] += c[k];
+      }
+      if (REPLAY) {
+        const float cs = dot(adj->dL, c[0]);
+        if (cs != 0.0f && any_nonzero(c[0])) {
+          Pcg32 r2 = rng_nee;
+          V3 em2[NS];
+          DirSample ds2;
+          nee_twostates<NS, true>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G);
+        }
+        adj->Lrem -= c[0];
+      }
+    }
+
+    // ---- BSDF sampling (:347-380)
+    BsdfSample bs;
+    bs.pdf = 0.0f;
+    bs.is_null = false;
+    bs.weight = v3(0.f);
+    bs.wo = v3(0.f, 0.f, 1.f);
+    V3 rho_new = v3(0.f);
+    if (active_surface) {
+      (void)rng.next_1d();
+      const float a = rng.next_1d();
+      const float b = rng.next_1d();
+      if (bsdf.type == RCV_BSDF_NULL) {
+        bs.wo = -wi;
+        bs.pdf = 1.0f;
+        bs.weight = v3(1.f);
+        bs.is_null = true;
+      } else if (bsdf.type == RCV_BSDF_DIFFUSE && wi.z > 0.0f) {
+        bs.wo = square_to_cosine_hemisphere(a, b);
+        bs.pdf = diffuse_pdf(wi, bs.wo);
+        rho_new = diffuse_reflectance(S, bsdf, state_index<NS>(NS - 1), si.u, si.v);
+        bs.weight = bs.pdf > 0.0f ? rho_new : v3(0.f);
+      }
+      active_surface = bs.pdf > 0.0f;
+    }
+    if (active_surface) {
+      if (NS == 2) {
+        // old state: f_old / pdf unless the sampled lobe is delta (null)
+        V3 w_old;
+        if (bs.is_null) {
+          w_old = bs.weight;
+        } else {
+          const V3 f_old = diffuse_eval(diffuse_reflectance(S, bsdf, RCV_STATE_OLD, si.u, si.v), wi, bs.wo);
+          w_old = f_old / bs.pdf;
+        }
+        thr[0] *= w_old;
+      }
+      thr[NS - 1] *= bs.weight;
+      const V3 d = bs.is_null ? ray.d : fr.to_world(bs.wo);
+      ray = spawn_ray(si, d);
+      needs_int = true;
+      if (!bs.is_null) {
+        depth += 1;
+        last_p = si.p;
+        last_pdf = bs.pdf;
+        cnt.c[kCntSurfaceHits]++;
+      }
+      if (!bs.is_null) specular_chain = false;  // diffuse lobe is smooth (:376-377)
+      const ShapeInfo sh = S.shapes[si.shape];
+      if (sh.interior != sh.exterior) medium = dot(si.n, ray.d) > 0.0f ? sh.exterior : sh.interior;
+    }
+    if (!(alive && (active_surface || active_medium))) break;
+  }
+}
+
+}  // namespace rcv
