@@ -1,0 +1,287 @@
+#!/usr/bin/env python3
+"""Benchmark of the recursive-control-variate inverse-rendering step on one
+MI355X.
+
+Workload (the heterogeneous-volume configuration): 64^3 float32 density grid
+in the unit cube (target: 8 Gaussian blobs from seed 1, sigma_t scale 20),
+albedo 0.8, isotropic phase, constant environment 1, 4 views of 256x256 on one
+batch film, 32 spp primal and 32 spp adjoint, max_depth 16, grid initialised at
+0.1, Adam lr 0.01, L1 loss, method cv_ps (two-state render + CV combine).
+
+A step = scene update (two-state pointer rotation) + two-state render + fused
+CV combine + loss/gradient + path-replay adjoint + Adam. ``value`` is
+Msamples/s = (W*H*spp primal paths + W*H*spp_grad adjoint paths) / step time.
+The JSON line also carries p50/p99 step ms, time-to-param-MSE 1e-3 for cv_ps
+and the plain (baseline) loop, the roofline of the dominant kernel and a CPU
+baseline (the float32 CPU restatement on host cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mse-steps", type=int, default=200, help="iterations per method for time-to-MSE (0: skip)")
+    ap.add_argument("--mse-threshold", type=float, default=1e-3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--spp", type=int, default=32)
+    ap.add_argument("--max-depth", type=int, default=16)
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    return ap.parse_args()
+
+
+def build_problem(args, method="cv_ps"):
+    import numpy as np
+    import torch
+    import recursive_control_variates_amd as rcv
+    from recursive_control_variates_amd import scenes
+    target = scenes.blob_grid(args.grid, 1)
+    init = np.full_like(target, 0.1)
+    sc = rcv.load_dict(scenes.volume_scene_dict(init, twostate=(method == "cv_ps"), film_res=args.res,
+                                                spp=args.spp))
+    itype = "twostateprbvolpath" if method == "cv_ps" else "prbvolpath"
+    integ = rcv.load_dict({"type": itype, "max_depth": args.max_depth, "rr_depth": args.max_depth})
+    return sc, integ, torch.from_numpy(target).cuda(), torch.from_numpy(init).cuda()
+
+
+def reference_image(args, sc, integ, target):
+    from recursive_control_variates_amd.optimize import render_reference
+    return render_reference({"ref_passes": 4, "spp_ref": 128}, sc, integ, target={
+        "object.interior_medium.sigma_t.data": target})
+
+
+class Stepper:
+    """One optimisation step of the workload, as the reference driver runs it
+    (optimize.py:104-129) with the meta integrator."""
+
+    def __init__(self, args, method="cv_ps", img_ref=None):
+        import torch
+        from recursive_control_variates_amd import scenes
+        from recursive_control_variates_amd.cv_integrator import MetaIntegrator
+        from recursive_control_variates_amd.optim import Adam, l1_loss
+        from recursive_control_variates_amd.scene import traverse
+        self.args = args
+        self.sc, self.base, self.target, init = build_problem(args, method)
+        self.key = scenes.VOLUME_KEY
+        self.img_ref = img_ref if img_ref is not None else reference_image(args, self.sc, self.base, self.target)
+        self.meta = MetaIntegrator({"method": method, "integrator": self.base})
+        self.params = traverse(self.sc)
+        self.opt = Adam(lr=0.01)
+        self.opt.set_clamp(self.key, 0.0, 1e9)
+        for _ in range(2 if method == "cv_ps" else 1):      # utils.py:62-87
+            self.opt[self.key] = init
+            self.params.update(self.opt)
+        self.loss_fn = l1_loss
+        self.i = 0
+        self.torch = torch
+
+    def step(self):
+        from recursive_control_variates_amd.rendering import render
+        self.params.update(self.opt)
+        img = render(self.sc, self.params, integrator=self.meta, seed=self.i, spp=self.args.spp,
+                     spp_grad=self.args.spp)
+        loss = self.loss_fn(img, self.img_ref)
+        loss.backward()
+        self.opt.step()
+        self.i += 1
+
+    def param_mse(self):
+        d = self.opt[self.key].detach() - self.target
+        return float((d * d).mean())
+
+
+def time_steps(st, steps, warmup):
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        st.step()
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    evs[0].record()
+    for k in range(steps):
+        st.step()
+        evs[k + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(steps)])
+    return wall, per
+
+
+def kernel_roofline(args, st):
+    """Times the two-state render (the dominant kernel) alone with HIP events
+    on the launch stream, and counts its events with device counters in a
+    separate launch so the counting atomics stay out of the timing."""
+    import torch
+    sc, integ = st.sc, st.base
+    for _ in range(2):
+        integ.render_twostates(sc, 0, 999, args.spp)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    ts = []
+    for r in range(args.kernel_reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        integ.render_twostates(sc, 0, 1000 + r, args.spp)
+        b.record(s)
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    sc.set_counting(True)
+    integ.render_twostates(sc, 0, 1000, args.spp)
+    c = sc.read_counters()
+    sc.set_counting(False)
+    n_pix = args.res * args.res * 4
+    # algorithmic bytes: 2 states x 8 trilinear corners x 4 B per tentative
+    # collision and per ratio-tracking step; 2 images x 12 B per pixel written
+    alg_bytes = 64 * (c["collisions"] + c["nee_steps"]) + 24 * n_pix
+    avg_ms = sum(ts) / len(ts)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_volume_twostate.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "kernel": "render_kernel<2, VOLPATH> (two-state delta tracking)",
+            "kernel_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg_bytes, "counters": c}
+
+
+def time_to_mse(args, img_ref):
+    import torch
+    out = {}
+    for method in ("cv_ps", "baseline"):
+        st = Stepper(args, method, img_ref=img_ref)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reached = None
+        mse = st.param_mse()
+        for i in range(args.mse_steps):
+            st.step()
+            if (i + 1) % 5 == 0 or i + 1 == args.mse_steps:
+                mse = st.param_mse()   # synchronises
+                if reached is None and mse <= args.mse_threshold:
+                    reached = (time.perf_counter() - t0, i + 1)
+        total = time.perf_counter() - t0
+        out[method] = {"seconds": round(reached[0], 3) if reached else None,
+                       "steps": reached[1] if reached else None,
+                       "final_mse": mse, "run_seconds": round(total, 3), "iterations": args.mse_steps}
+        log(f"[bench] time-to-mse {method}: {out[method]}")
+        del st
+        torch.cuda.empty_cache()
+    a, b = out["cv_ps"]["seconds"], out["baseline"]["seconds"]
+    out["speedup_cv_vs_plain"] = round(b / a, 3) if (a and b) else None
+    return out
+
+
+def cpu_baseline(args):
+    """The float32 CPU restatement (oracle) timed on host cores on a bounded
+    slice of the workload's pixels: two-state primal paths per second."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import render_ref as R
+    from recursive_control_variates_amd import scene_desc, scenes
+    target = scenes.blob_grid(args.grid, 1)
+    init = np.full_like(target, 0.1)
+    d = scene_desc.describe(scenes.volume_scene_dict(init, film_res=args.res, spp=args.spp))
+    threads = min(16, os.cpu_count() or 1)
+    n_pix_total = d.sensors[0].film_width * d.sensors[0].film_height
+    # middle band of the film (rays through the medium), growing until the budget is used
+    n = 256
+    spent = 0.0
+    while True:
+        b = n_pix_total // 2 - n // 2
+        t0 = time.perf_counter()
+        R.render(d, 1, args.max_depth, args.max_depth, 7, args.spp, params_old=[target], params_new=[init],
+                 precision=32, n_threads=threads, pixel_range=(b, b + n))
+        dt = time.perf_counter() - t0
+        spent += dt
+        if dt > args.cpu_seconds / 3 or spent > args.cpu_seconds or n >= n_pix_total:
+            break
+        n = min(n_pix_total, n * 4)
+    msps = n * args.spp / dt / 1e6
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(msps, 4), "unit": "Msamples/s (two-state primal paths)", "cores": threads,
+            "kind": "port", "sample": f"{n} pixels x {args.spp} spp of the 4x256^2 film (central band), "
+                                      f"two-state primal, float32, {dt:.2f}s on {cpu}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    st = Stepper(args, "cv_ps")
+    log("[bench] reference image rendered")
+    wall, per = time_steps(st, args.steps, args.warmup)
+    ms = wall * 1e3 / args.steps
+    n_pix = args.res * args.res * 4
+    samples = n_pix * args.spp * 2
+    msps = samples / (ms * 1e-3) / 1e6
+    log(f"[bench] {args.steps} steps: {ms:.3f} ms/step, {msps:.1f} Msamples/s")
+    roof = kernel_roofline(args, st)
+    log(f"[bench] roofline {roof}")
+    img_ref = st.img_ref
+    del st
+    ttm = time_to_mse(args, img_ref) if args.mse_steps > 0 else None
+    cpu = cpu_baseline(args) if args.cpu_seconds > 0 else None
+    log(f"[bench] cpu baseline {cpu}")
+    out = {
+        "metric": "Msamples/s (two-state primal + adjoint paths) of a cv_ps volume optimisation step; "
+                  "also time-to-param-MSE 1e-3 (CV vs plain) and p50/p99 step ms",
+        "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (procedural blob-grid target, seed 1)",
+        "config": {"workload": f"heterogeneous volume {args.grid}^3, 4 views {args.res}x{args.res}, "
+                               f"spp {args.spp}/{args.spp}, max_depth {args.max_depth}, cv_ps, Adam lr 0.01, L1",
+                   "global_batch": n_pix * args.spp, "parallelism": "single GPU"},
+        "p50_step_ms": round(float(np.percentile(per, 50)), 4),
+        "p99_step_ms": round(float(np.percentile(per, 99)), 4),
+        "time_to_param_mse": ttm,
+        "roofline": {k: v for k, v in roof.items() if k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_detail": {k: v for k, v in roof.items() if k not in ("bound", "achieved", "peak", "unit", "frac",
+                                                                      "traffic")},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,7 +67,7 @@ def traverse(scene):
 
 
 def render(scene, params=None, sensor=0, integrator=None, seed=0, seed_grad=0, spp=0, spp_grad=0):
-    from .render import render as _render
+    from .rendering import render as _render
     return _render(scene, params, sensor, integrator, seed, seed_grad, spp, spp_grad)
 
 

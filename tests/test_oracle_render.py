@@ -8,7 +8,7 @@ import pytest
 import render_ref as R
 from helpers import (box_chord, camera_rays, floor_light_scene, furnace_box_scene, volume_box_scene)
 from recursive_control_variates_amd import scene_desc
-from recursive_control_variates_amd.render import sample_tea_32
+from recursive_control_variates_amd.rendering import sample_tea_32
 
 VOL, PATH = 1, 0
 
