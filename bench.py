@@ -127,73 +127,114 @@ def time_steps(st, steps, warmup):
     return wall, per
 
 
-def kernel_roofline(args, st):
-    """Times the two-state render (the dominant kernel) alone with HIP events
-    on the launch stream, and counts its events with device counters in a
-    separate launch so the counting atomics stay out of the timing."""
+def _event_time(fn, reps):
+    """Average duration (ms) of fn's launches, HIP events on the launch stream."""
     import torch
-    sc, integ = st.sc, st.base
-    for _ in range(2):
-        integ.render_twostates(sc, 0, 999, args.spp)
-    torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     ts = []
-    for r in range(args.kernel_reps):
+    for r in range(reps):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record(s)
-        integ.render_twostates(sc, 0, 1000 + r, args.spp)
+        fn(r)
         b.record(s)
         torch.cuda.synchronize()
         ts.append(a.elapsed_time(b))
-    sc.set_counting(True)
-    integ.render_twostates(sc, 0, 1000, args.spp)
-    c = sc.read_counters()
-    sc.set_counting(False)
-    n_pix = args.res * args.res * 4
-    # algorithmic bytes: 2 states x 8 trilinear corners x 4 B per tentative
-    # collision and per ratio-tracking step; 2 images x 12 B per pixel written
-    alg_bytes = 64 * (c["collisions"] + c["nee_steps"]) + 24 * n_pix
-    avg_ms = sum(ts) / len(ts)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_volume_twostate.json")
+    return sum(ts) / len(ts)
+
+
+def _traffic(name):
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(tpath):
         try:
             with open(tpath) as f:
-                traffic = json.load(f).get("bytes_per_launch")
+                return json.load(f).get("bytes_per_launch")
         except Exception:
-            traffic = None
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "kernel": "render_kernel<2, VOLPATH> (two-state delta tracking)",
-            "kernel_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg_bytes, "counters": c}
+            return None
+    return None
+
+
+def kernel_roofline(args, st):
+    """Rooflines of the step's two render kernels, each timed alone with HIP
+    events on the launch stream; event counts come from device counters in a
+    separate launch so the counting atomics stay out of the timing. The
+    adjoint is the dominant kernel of a step."""
+    import torch
+    sc, integ = st.sc, st.base
+    n_pix = args.res * args.res * 4
+    # two-state render: 2 states x 8 trilinear corners x 4 B per tentative
+    # collision and per ratio-tracking step; 2 images x 12 B per pixel
+    integ.render_twostates(sc, 0, 999, args.spp)
+    torch.cuda.synchronize()
+    r_ms = _event_time(lambda r: integ.render_twostates(sc, 0, 1000 + r, args.spp), args.kernel_reps)
+    sc.set_counting(True)
+    integ.render_twostates(sc, 0, 1000, args.spp)
+    rc = sc.read_counters()
+    r_bytes = 64 * (rc["collisions"] + rc["nee_steps"]) + 24 * n_pix
+    # adjoint: 1 state x 8 corners x 4 B per lookup in the primal, replay and
+    # NEE-gradient passes; 4 B per gradient scatter-add; 12 B per pixel read
+    gimg = torch.full(sc.film_shape(0), 1.0 / (n_pix * 3), device="cuda")
+    st.params.update(st.opt)
+    sc.set_counting(False)
+    integ.adjoint(sc, st.params, gimg, 0, 77, args.spp)
+    torch.cuda.synchronize()
+    a_ms = _event_time(lambda r: integ.adjoint(sc, st.params, gimg, 0, 2000 + r, args.spp), args.kernel_reps)
+    sc.set_counting(True)
+    integ.adjoint(sc, st.params, gimg, 0, 2000, args.spp)
+    ac = sc.read_counters()
+    sc.set_counting(False)
+    a_bytes = 32 * (ac["collisions"] + ac["nee_steps"]) + 4 * ac["grad_atomics"] + 12 * n_pix
+    out = {}
+    for name, ms, by, cnt, kern in (("volume_adjoint", a_ms, a_bytes, ac, "adjoint_kernel<VOLPATH> (path replay)"),
+                                    ("volume_twostate", r_ms, r_bytes, rc,
+                                     "render_kernel<2, VOLPATH> (two-state delta tracking)")):
+        ach = by / (ms * 1e-3) / 1e9
+        out[name] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": _traffic(name), "kernel": kern,
+                     "kernel_ms": round(ms, 4), "alg_bytes_per_launch": by, "counters": cnt}
+    return out
+
+
+MSE_LEVELS = (1e-3, 2e-2, 1.5e-2, 1.25e-2)
 
 
 def time_to_mse(args, img_ref):
+    """Wall-clock until the unscaled-grid MSE reaches each level, for cv_ps and
+    the plain loop (MSE checked every 5 iterations; the check synchronises)."""
     import torch
+    levels = sorted(set(MSE_LEVELS) | {args.mse_threshold}, reverse=True)
     out = {}
     for method in ("cv_ps", "baseline"):
         st = Stepper(args, method, img_ref=img_ref)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        reached = None
+        reached = {}
         mse = st.param_mse()
+        best = mse
         for i in range(args.mse_steps):
             st.step()
             if (i + 1) % 5 == 0 or i + 1 == args.mse_steps:
-                mse = st.param_mse()   # synchronises
-                if reached is None and mse <= args.mse_threshold:
-                    reached = (time.perf_counter() - t0, i + 1)
+                mse = st.param_mse()
+                best = min(best, mse)
+                for lv in levels:
+                    if lv not in reached and mse <= lv:
+                        reached[lv] = (round(time.perf_counter() - t0, 3), i + 1)
         total = time.perf_counter() - t0
-        out[method] = {"seconds": round(reached[0], 3) if reached else None,
-                       "steps": reached[1] if reached else None,
-                       "final_mse": mse, "run_seconds": round(total, 3), "iterations": args.mse_steps}
+        out[method] = {"seconds": reached.get(args.mse_threshold, (None,))[0],
+                       "by_level": {f"{lv:g}": {"seconds": reached[lv][0], "steps": reached[lv][1]}
+                                    if lv in reached else None for lv in levels},
+                       "final_mse": mse, "best_mse": best, "run_seconds": round(total, 3),
+                       "ms_per_iteration": round(total * 1e3 / max(1, args.mse_steps), 3),
+                       "iterations": args.mse_steps}
         log(f"[bench] time-to-mse {method}: {out[method]}")
         del st
         torch.cuda.empty_cache()
-    a, b = out["cv_ps"]["seconds"], out["baseline"]["seconds"]
-    out["speedup_cv_vs_plain"] = round(b / a, 3) if (a and b) else None
+    ratios = {}
+    for lv in levels:
+        a = out["cv_ps"]["by_level"][f"{lv:g}"]
+        b = out["baseline"]["by_level"][f"{lv:g}"]
+        ratios[f"{lv:g}"] = round(b["seconds"] / a["seconds"], 3) if (a and b and a["seconds"]) else None
+    out["speedup_cv_vs_plain"] = ratios
     return out
 
 
@@ -275,9 +316,9 @@ def main():
         "p50_step_ms": round(float(np.percentile(per, 50)), 4),
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
         "time_to_param_mse": ttm,
-        "roofline": {k: v for k, v in roof.items() if k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
-        "roofline_detail": {k: v for k, v in roof.items() if k not in ("bound", "achieved", "peak", "unit", "frac",
-                                                                      "traffic")},
+        "roofline": {k: v for k, v in roof["volume_adjoint"].items()
+                     if k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_detail": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -222,6 +222,57 @@ int rcv_adam_step(float *param_out, const float *param_in, const float *grad,
                   float beta1, float beta2, float eps, float clamp_lo,
                   float clamp_hi, void *stream);
 
+/* ------------------------------------------------------ batched jobs
+ * Many independent optimisations (each: one committed scene of its own, one
+ * optimised parameter, a method, its CV state and Adam state) advanced by one
+ * device-resident iteration: two-state/plain render in persistent work-queue
+ * kernels over all jobs, CV combine, loss gradient, path-replay adjoint, Adam
+ * + clamp, two-state pointer rotation. Seeds (job seed + iteration), Adam step
+ * counts and majorants advance on the device, so rcv_batch_step can capture
+ * the iteration into a hipGraph once and replay it. Multi-job form of
+ * run_opt's loop body (optimize.py:104-129); no reference counterpart exists
+ * for the batching itself. All buffers are caller-owned device memory. */
+typedef struct rcv_batch rcv_batch;
+
+typedef struct rcv_batch_job {
+  const rcv_scene *scene;          /* committed, used by this job only */
+  rcv_integrator_desc integrator;
+  int32_t sensor;
+  uint32_t seed;                   /* iteration i renders with seed + i */
+  int32_t spp, spp_grad;
+  int32_t method;                  /* 0 = baseline (plain), 1 = cv_ps */
+  int32_t loss;                    /* RCV_LOSS_* */
+  int32_t param;                   /* optimised parameter id of `scene` */
+  float *param_ring[3];            /* [0] old, [1] new (both = initial value), [2] scratch */
+  float *adam_m, *adam_v, *grad;   /* parameter-sized, m = v = 0 initially */
+  float lr, beta1, beta2, eps, clamp_lo, clamp_hi;
+  const float *img_ref;            /* film-sized reference image */
+  float *H, *F, *img, *grad_img;   /* film-sized work images */
+  float *cv_state[8];              /* film-sized, zero: img v_n w_s mean_x mean_y var_x var_y cov */
+  int32_t warmup;
+  float cv_beta1, cv_beta2;
+  int32_t pre_update;
+} rcv_batch_job;
+
+int rcv_batch_create(rcv_batch **out);
+int rcv_batch_destroy(rcv_batch *batch);
+int rcv_batch_add_job(rcv_batch *batch, const rcv_batch_job *job, int32_t *out_job);
+/* Lays the jobs out in device memory; the scenes' parameter states must be
+ * bound to param_ring[0] (OLD) and param_ring[1] (NEW) beforehand. */
+int rcv_batch_commit(rcv_batch *batch, void *stream);
+/* Launch shape of the persistent kernels: threads per block (64..512, a
+ * multiple of 64) and work units taken per queue grab (64..8192, a multiple
+ * of 64); 0 = defaults. Invalidates a captured graph. */
+int rcv_batch_set_launch(rcv_batch *batch, int32_t block_size, int32_t queue_grab);
+/* One iteration of every job on `stream` (must not be the legacy default
+ * stream when use_graph != 0: the first call captures a hipGraph, later calls
+ * replay it). */
+int rcv_batch_step(rcv_batch *batch, int32_t use_graph, void *stream);
+/* Iteration count of a job and the ring slot holding its latest parameter
+ * value (synchronises `stream`). */
+int rcv_batch_job_state(const rcv_batch *batch, int32_t job, int32_t *iteration, int32_t *ring_slot,
+                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,7 @@ BUILD_DIR = os.path.join(PKG_DIR, "_build")
 LIB_PATH = os.path.join(PKG_DIR, "librcv.so")
 ARCH = "gfx950"
 
-SOURCES = ["combine.hip", "optim.hip", "scene.hip", "render.hip"]
+SOURCES = ["combine.hip", "optim.hip", "scene.hip", "render.hip", "batch.hip"]
 
 COMMON_FLAGS = [
     "-O3",

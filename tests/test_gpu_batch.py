@@ -1,0 +1,125 @@
+"""Batched optimisations (rcv_batch_*): a batch of one job follows the
+single-job driver; a job's results are the same alone and in a mix of
+uneven jobs; hipGraph replay matches eager launches; the autotuner runs."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce
+from helpers import furnace_box_scene, volume_box_scene
+import recursive_control_variates_amd as rcv
+from recursive_control_variates_amd.batch import BatchOptimizer, JobSpec, autotune
+from recursive_control_variates_amd.cv_integrator import MetaIntegrator
+from recursive_control_variates_amd.optim import Adam, l1_loss
+from recursive_control_variates_amd.rendering import render
+
+pytestmark = pytest.mark.gpu
+VKEY = "box.interior_medium.sigma_t.data"
+TKEY = "back.bsdf.reflectance.data"
+
+
+def vol_spec(seed, n=6, w=12, h=10, spp=8, method="cv_ps", max_depth=8, twostate=True):
+    rng = np.random.default_rng(seed)
+    target = rng.uniform(0.2, 1.0, (n, n, n, 1)).astype(np.float32)
+    init = np.full_like(target, 0.5)
+    d = volume_box_scene(init, scale=5.0, w=w, h=h, spp=spp, twostate=twostate)
+    sc = rcv.load_dict(d)
+    p = rcv.traverse(sc)
+    p[VKEY] = torch.from_numpy(target).cuda()
+    p.update()
+    p.update()
+    ref = rcv.load_dict({"type": "prbvolpath", "max_depth": max_depth, "rr_depth": max_depth}).render(sc, 0, 999, 64)
+    p[VKEY] = torch.from_numpy(init).cuda()
+    p.update()
+    p.update()
+    return JobSpec(scene=sc, key=VKEY, init=init, img_ref=ref, method=method, spp=spp, seed=0,
+                   integrator={"type": "prbvolpath", "max_depth": max_depth, "rr_depth": max_depth},
+                   lr=0.02, clamp=(0.0, 1e9))
+
+
+def surf_spec(seed, w=10, h=10, spp=8, method="cv_ps"):
+    rng = np.random.default_rng(seed)
+    tex = rng.uniform(0.2, 0.8, (4, 4, 3)).astype(np.float32)
+    sc = rcv.load_dict(furnace_box_scene(albedo=0.5, w=w, h=h, spp=spp, textured=True, tex=tex))
+    ref = torch.from_numpy(rng.uniform(1.0, 2.0, (h, w, 3)).astype(np.float32)).cuda()
+    init = np.full((4, 4, 3), 0.5, np.float32)
+    return JobSpec(scene=sc, key=TKEY, init=init, img_ref=ref, method=method, spp=spp, seed=0,
+                   integrator={"type": "prb", "max_depth": 3, "rr_depth": 10}, lr=0.02, clamp=(0.0, 1.0))
+
+
+def driver_run(spec, steps):
+    """The single-job path: meta integrator + autograd render + Adam."""
+    sc = spec.scene
+    base = rcv.load_dict(dict(spec.integrator, type="twostateprbvolpath" if spec.method == "cv_ps"
+                              else "prbvolpath"))
+    meta = MetaIntegrator({"method": spec.method, "integrator": base})
+    p = rcv.traverse(sc)
+    opt = Adam(lr=spec.lr)
+    opt.set_clamp(spec.key, *spec.clamp)
+    for _ in range(2 if spec.method == "cv_ps" else 1):
+        opt[spec.key] = torch.as_tensor(spec.init).cuda()
+        p.update(opt)
+    imgs = []
+    for i in range(steps):
+        p.update(opt)
+        img = render(sc, p, integrator=meta, seed=i, spp=spec.spp, spp_grad=spec.spp)
+        imgs.append(img.detach().clone())
+        l1_loss(img, spec.img_ref).backward()
+        opt.step()
+    return opt[spec.key].detach(), imgs
+
+
+def test_batch_of_one_follows_driver(cuda):
+    announce("batch of one vs driver")
+    spec = vol_spec(1)
+    ref_param, ref_imgs = driver_run(vol_spec(1), 3)
+    b = BatchOptimizer([spec], use_graph=False)
+    b.step(1)
+    np.testing.assert_allclose(b.image(0).cpu().numpy(), ref_imgs[0].cpu().numpy(), rtol=1e-5, atol=1e-6)
+    b.step(2)
+    np.testing.assert_allclose(b.param(0).cpu().numpy(), ref_param.cpu().numpy(), rtol=1e-3, atol=1e-4)
+    it, _ = b.job_state(0)
+    assert it == 3
+
+
+def test_same_alone_and_in_the_mix(cuda):
+    announce("alone vs mix")
+    mk = [lambda: vol_spec(2, n=6, w=12, h=10, spp=8),
+          lambda: vol_spec(3, n=8, w=20, h=8, spp=4, method="baseline", twostate=False),
+          lambda: surf_spec(4, w=16, h=12, spp=8),
+          lambda: vol_spec(5, n=4, w=8, h=8, spp=16, max_depth=12)]
+    alone = BatchOptimizer([mk[0]()], use_graph=False)
+    mix = BatchOptimizer([f() for f in mk], use_graph=False)
+    alone.step(1)
+    mix.step(1)
+    # first iteration: identical images (same random streams, same kernels)
+    assert torch.equal(alone.image(0), mix.image(0))
+    alone.step(3)
+    mix.step(3)
+    a = alone.param(0).cpu().numpy()
+    m = mix.param(0).cpu().numpy()
+    # later iterations differ only through float atomics' summation order
+    np.testing.assert_allclose(a, m, rtol=1e-3, atol=1e-4)
+    for j in range(4):
+        assert torch.isfinite(mix.param(j)).all()
+
+
+def test_graph_replay_matches_eager(cuda):
+    announce("graph vs eager")
+    specs = lambda: [vol_spec(6, w=12, h=12, spp=8), surf_spec(7, spp=4), vol_spec(8, n=5, spp=4)]
+    eager = BatchOptimizer(specs(), use_graph=False)
+    graph = BatchOptimizer(specs(), use_graph=True)
+    eager.step(4)
+    graph.step(4)
+    for j in range(3):
+        np.testing.assert_allclose(graph.param(j).cpu().numpy(), eager.param(j).cpu().numpy(),
+                                   rtol=1e-3, atol=1e-4)
+        assert graph.job_state(j)[0] == 4
+
+
+def test_autotune(cuda):
+    announce("autotune")
+    best, table = autotune(lambda: [vol_spec(9, spp=4), surf_spec(10, spp=4)], blocks=(64, 256),
+                           grabs=(64, 512), steps=2)
+    assert best in table and len(table) == 4
+    assert all(v > 0 for v in table.values())

@@ -75,7 +75,10 @@ def test_per_pixel_three_sigma_vs_fp64(cuda):
         z = (mean - o) / se
         assert np.mean(np.abs(z) > 3) < 0.02, np.mean(np.abs(z) > 3)
         assert np.abs(z).max() < 6
-        assert abs(z.mean()) < 0.25
+        # grey medium: the 3 channels of a pixel are one sample, so 80
+        # independent z values; their mean has standard deviation 1/sqrt(80)
+        n_eff = z.shape[0] * z.shape[1]
+        assert abs(z[..., 0].mean()) < 4.0 / np.sqrt(n_eff)
 
 
 def test_batch_sensor_volume_workload_shape(cuda):
