@@ -46,7 +46,27 @@ def parse():
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--profile-target", action="store_true",
+                    help="run only: 3 workload steps, one two-state render, one adjoint and a 256 MiB copy "
+                         "(the calibration kernel), for counter collection")
     return ap.parse_args()
+
+
+def profile_target(args):
+    import torch
+    st = Stepper(args, "cv_ps")
+    for _ in range(3):
+        st.step()
+    torch.cuda.synchronize()
+    st.base.render_twostates(st.sc, 0, 5000, args.spp)
+    st.params.update(st.opt)
+    g = torch.full(st.sc.film_shape(0), 1e-6, device="cuda")
+    st.base.adjoint(st.sc, st.params, g, 0, 6000, args.spp)
+    x = torch.ones(64 * 1024 * 1024, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log("[bench] profile target done")
 
 
 def build_problem(args, method="cv_ps"):
@@ -289,6 +309,9 @@ def main():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    if args.profile_target:
+        profile_target(args)
+        return
     st = Stepper(args, "cv_ps")
     log("[bench] reference image rendered")
     wall, per = time_steps(st, args.steps, args.warmup)

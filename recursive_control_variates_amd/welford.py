@@ -71,7 +71,7 @@ class StatisticsEstimator:
 
     def get(self):
         """(var_x, var_y, cov) / (1 - beta2^n) (plugins/welford.py:77-78)."""
-        b = _bias(self.beta2, int(self.n.item()))
+        b = torch.tensor(_bias(self.beta2, int(self.n.item())), dtype=torch.float32, device=self.var_x.device)
         return self.var_x / b, self.var_y / b, self.cov / b
 
 
@@ -101,4 +101,4 @@ class WelfordVarianceEstimator:
 
     def get(self):
         n = int(self.counter[0].item()) if self.counter is not None else 0
-        return self.var / _bias(self.beta2, n)
+        return self.var / torch.tensor(_bias(self.beta2, n), dtype=torch.float32, device=self.var.device)

@@ -82,24 +82,38 @@ def test_batch_of_one_follows_driver(cuda):
     assert it == 3
 
 
+def _grads_close(a, b):
+    a = a.cpu().numpy()
+    b = b.cpu().numpy()
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-5 * max(np.abs(b).max(), 1e-12))
+
+
+def _params_close(a, b, lr, steps):
+    # float atomics make gradient sums order-dependent in the last bits; Adam's
+    # early steps are ~lr * sign(g), so an element whose gradient is near 0
+    # may move by up to 2 lr per step differently; the bulk stays identical
+    d = np.abs(a.cpu().numpy() - b.cpu().numpy())
+    assert d.max() <= 2 * lr * steps + 1e-6, d.max()
+    assert np.median(d) <= 1e-4, np.median(d)
+
+
 def test_same_alone_and_in_the_mix(cuda):
     announce("alone vs mix")
     mk = [lambda: vol_spec(2, n=6, w=12, h=10, spp=8),
           lambda: vol_spec(3, n=8, w=20, h=8, spp=4, method="baseline", twostate=False),
           lambda: surf_spec(4, w=16, h=12, spp=8),
           lambda: vol_spec(5, n=4, w=8, h=8, spp=16, max_depth=12)]
-    alone = BatchOptimizer([mk[0]()], use_graph=False)
-    mix = BatchOptimizer([f() for f in mk], use_graph=False)
-    alone.step(1)
-    mix.step(1)
-    # first iteration: identical images (same random streams, same kernels)
-    assert torch.equal(alone.image(0), mix.image(0))
-    alone.step(3)
-    mix.step(3)
-    a = alone.param(0).cpu().numpy()
-    m = mix.param(0).cpu().numpy()
-    # later iterations differ only through float atomics' summation order
-    np.testing.assert_allclose(a, m, rtol=1e-3, atol=1e-4)
+    for j in (0, 2):
+        alone = BatchOptimizer([mk[j]()], use_graph=False)
+        mix = BatchOptimizer([f() for f in mk], use_graph=False)
+        alone.step(1)
+        mix.step(1)
+        # first iteration: identical images (same random streams, same kernels)
+        assert torch.equal(alone.image(0), mix.image(j))
+        _grads_close(mix.jobs[j].grad, alone.jobs[0].grad)
+        alone.step(3)
+        mix.step(3)
+        _params_close(alone.param(0), mix.param(j), 0.02, 4)
     for j in range(4):
         assert torch.isfinite(mix.param(j)).all()
 
@@ -109,11 +123,17 @@ def test_graph_replay_matches_eager(cuda):
     specs = lambda: [vol_spec(6, w=12, h=12, spp=8), surf_spec(7, spp=4), vol_spec(8, n=5, spp=4)]
     eager = BatchOptimizer(specs(), use_graph=False)
     graph = BatchOptimizer(specs(), use_graph=True)
-    eager.step(4)
-    graph.step(4)
+    eager.step(1)
+    graph.step(1)
+    graph.synchronize()
+    eager.synchronize()
     for j in range(3):
-        np.testing.assert_allclose(graph.param(j).cpu().numpy(), eager.param(j).cpu().numpy(),
-                                   rtol=1e-3, atol=1e-4)
+        assert torch.equal(graph.jobs[j].img, eager.jobs[j].img)
+        _grads_close(graph.jobs[j].grad, eager.jobs[j].grad)
+    eager.step(3)
+    graph.step(3)
+    for j in range(3):
+        _params_close(graph.param(j), eager.param(j), 0.02, 4)
         assert graph.job_state(j)[0] == 4
 
 
