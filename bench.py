@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--mode", choices=("step", "batch", "serve"), default="step",
+                    help="step: the volume optimisation step (default line); batch: 64 concurrent jobs in one "
+                         "graph-replayed iteration; serve: continuous batching under Poisson step requests")
+    ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--profile-target", action="store_true",
                     help="run only: 3 workload steps, one two-state render, one adjoint and a 256 MiB copy "
                          "(the calibration kernel), for counter collection")
@@ -298,6 +303,83 @@ def cpu_baseline(args):
                                       f"two-state primal, float32, {dt:.2f}s on {cpu}"}
 
 
+def bench_batch(args):
+    """Config-4 style: ``--jobs`` volume optimisations (per-job seeds 100.., image
+    side U{64..512}, grid U{32,64,128}, spp U{4..64}, max_depth U{8..32}), cv_ps,
+    advanced together by one hipGraph-replayed batched iteration, after an
+    autotuner sweep of block size x queue grab; compared with the plain per-job
+    GPU loop (baseline method, one job at a time through the single-job path)."""
+    import numpy as np
+    import torch
+    from recursive_control_variates_amd.batch import BatchOptimizer, autotune
+    from recursive_control_variates_amd.cv_integrator import MetaIntegrator
+    from recursive_control_variates_amd.optim import Adam, l1_loss
+    from recursive_control_variates_amd.rendering import render
+    from recursive_control_variates_amd.scene import traverse
+    from recursive_control_variates_amd.serve import make_job
+    import recursive_control_variates_amd as rcv
+    seeds = list(range(100, 100 + args.jobs))
+    t0 = time.perf_counter()
+    made = [make_job(s, "volume", "cv_ps") for s in seeds]
+    log(f"[bench] {len(made)} jobs set up in {time.perf_counter() - t0:.1f}s")
+    paths = sum((m["side"] ** 2) * m["spp"] * 2 for _, m in made)
+    small = lambda: [make_job(s, "volume", "cv_ps")[0] for s in seeds[:8]]
+    best, table = autotune(small, blocks=(128, 256, 512), grabs=(256, 1024, 4096), steps=2)
+    log(f"[bench] autotune best {best}: {table}")
+    opt = BatchOptimizer([sp for sp, _ in made], use_graph=True, block_size=best[0], queue_grab=best[1])
+    opt.step(2)
+    opt.synchronize()
+    t = time.perf_counter()
+    K = max(1, args.steps // 4)
+    opt.step(K)
+    opt.synchronize()
+    batched_ms = (time.perf_counter() - t) * 1e3 / K
+    log(f"[bench] batched iteration of {len(seeds)} jobs: {batched_ms:.1f} ms")
+    opt.close()
+    # plain per-job loop: each job one baseline iteration through the single-job path
+    per_job = []
+    for s in seeds:
+        spec, m = make_job(s, "volume", "baseline")
+        base = rcv.load_dict(dict(spec.integrator))
+        meta = MetaIntegrator({"method": "baseline", "integrator": base})
+        p = traverse(spec.scene)
+        ad = Adam(lr=spec.lr)
+        ad.set_clamp(spec.key, *spec.clamp)
+        ad[spec.key] = torch.as_tensor(spec.init).cuda()
+        p.update(ad)
+
+        def one(i):
+            p.update(ad)
+            img = render(spec.scene, p, integrator=meta, seed=i, spp=spec.spp, spp_grad=spec.spp)
+            l1_loss(img, spec.img_ref).backward()
+            ad.step()
+        one(0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        one(1)
+        one(2)
+        torch.cuda.synchronize()
+        per_job.append((time.perf_counter() - t) * 1e3 / 2)
+    seq_ms = float(sum(per_job))
+    return {"metric": "batched cv_ps iteration of heterogeneous volume jobs vs plain per-job GPU loop",
+            "jobs": len(seeds), "batched_ms_per_iteration": round(batched_ms, 3),
+            "plain_per_job_loop_ms_per_iteration": round(seq_ms, 3),
+            "speedup": round(seq_ms / batched_ms, 3), "msamples_per_s": round(paths / batched_ms / 1e3, 3),
+            "autotune": {"best": list(best), "ms": {f"{b}x{g}": round(v, 3) for (b, g), v in table.items()}},
+            "hip_graph": True, "dtype": "f32", "data": "synthetic (blob targets, seeds 100..)"}
+
+
+def bench_serve(args):
+    """Config-5 style: Poisson step requests at 200/s over a 70% surface /
+    30% volume job mix (sizes as config 4, 100-500 iterations per job), HBM
+    admission budget 200 GB, continuous batching."""
+    from recursive_control_variates_amd.serve import run_simulation
+    out = run_simulation(duration=args.serve_seconds, rate=200.0, surface_fraction=0.7, target_jobs=16,
+                         iterations=(100, 500), budget_bytes=200e9, seed=0, log=log)
+    out["metric"] = "served optimisation steps: p50/p99 step latency and Msamples/s"
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -311,6 +393,12 @@ def main():
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
     if args.profile_target:
         profile_target(args)
+        return
+    if args.mode == "batch":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.mode == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
         return
     st = Stepper(args, "cv_ps")
     log("[bench] reference image rendered")

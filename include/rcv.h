@@ -175,6 +175,16 @@ int rcv_render_backward(const rcv_scene *scene,
 int rcv_scene_set_counting(rcv_scene *scene, int32_t enabled);
 int rcv_scene_read_counters(const rcv_scene *scene, int64_t *out, int32_t n,
                             void *stream);
+/* Majorant supergrid of medium `medium` for one state (replaces
+ * Medium.majorant_grid(), read at plugins/twostatemedium.py:25-29,52-53,
+ * 82-84): writes the resolution (x, y, z) to `res` ((0, 0, 0) when the medium
+ * uses one global majorant) and, when `out` is not null, copies the unscaled
+ * cell maxima, x fastest, to host memory `out` of `capacity` floats
+ * (synchronises `stream`). A medium created with majorant_factor k > 0 has
+ * ceil(n / k) cells per axis of an n-voxel grid; each cell holds the maximum
+ * of every voxel a trilinear lookup inside the cell reads. */
+int rcv_scene_majorant_grid(const rcv_scene *scene, int32_t medium, int32_t state,
+                            float *out, int64_t capacity, int32_t *res, void *stream);
 
 /* --------------------------------------------------------------- CV state
  * Per-pixel control-variate state of CVIntegrator / MetaIntegrator
@@ -252,7 +262,16 @@ typedef struct rcv_batch_job {
   int32_t warmup;
   float cv_beta1, cv_beta2;
   int32_t pre_update;
+  /* resume != 0: continue a job taken from another batch (rcv_batch_read_job)
+   * with this iteration count, ring slots and pending majorant key;
+   * otherwise start at iteration 0 with slots (0, 1, 2). */
+  int32_t resume;
+  int32_t iteration;
+  int32_t ring_old, ring_new, ring_next;
+  uint32_t next_key;
 } rcv_batch_job;
+
+#define RCV_BATCH_MAX_JOBS 4096
 
 int rcv_batch_create(rcv_batch **out);
 int rcv_batch_destroy(rcv_batch *batch);
@@ -272,6 +291,13 @@ int rcv_batch_step(rcv_batch *batch, int32_t use_graph, void *stream);
  * value (synchronises `stream`). */
 int rcv_batch_job_state(const rcv_batch *batch, int32_t job, int32_t *iteration, int32_t *ring_slot,
                         void *stream);
+/* Continuous batching: bit j of `active_bits` (uint32 words) selects whether
+ * job j takes part in the next iterations; inactive jobs keep their state
+ * untouched. Enqueued on `stream` (outside any captured graph). */
+int rcv_batch_set_active(rcv_batch *batch, const uint32_t *active_bits, int32_t n_jobs, void *stream);
+/* The resumable state of a job (see rcv_batch_job.resume); synchronises. */
+int rcv_batch_read_job(const rcv_batch *batch, int32_t job, int32_t *iteration, int32_t *ring_old,
+                       int32_t *ring_new, int32_t *ring_next, uint32_t *next_key, void *stream);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@
 // comparisons and as the timed CPU baseline. A plain render is the NEW image
 // of a two-state render whose two states are equal.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -63,6 +64,7 @@ typedef struct OracleScene {
   const float *const *param_new;
   int film_w, film_h, n_views;
   const float *views;  // origin3 left3 up3 fwd3 tan_half  (13 per view)
+  const int *med_majorant_factor;  // per medium: 0 = one global majorant, k > 0 = supergrid of ceil(n / k) cells
 } OracleScene;
 
 typedef struct OracleIntegrator {
@@ -243,6 +245,10 @@ struct Scene {
   V<R> bs_c;
   R bs_r = 0;
   std::vector<R> maj_old, maj_new;  // per param, unscaled max
+  // per medium: supergrid resolution (0 = none) and unscaled cell maxima of
+  // both states
+  std::vector<std::array<int, 3>> sg_res;
+  std::vector<std::vector<R>> sg_old, sg_new;
 
   explicit Scene(const OracleScene &sc) : s(sc) {
     for (int t = 0; t < s.n_tris; ++t) {
@@ -309,6 +315,107 @@ struct Scene {
       maj_old.push_back(R(mo));
       maj_new.push_back(R(mn));
     }
+    sg_res.assign(s.n_media, {0, 0, 0});
+    sg_old.resize(s.n_media);
+    sg_new.resize(s.n_media);
+    for (int m = 0; m < s.n_media; ++m) {
+      const int k = s.med_majorant_factor ? s.med_majorant_factor[m] : 0;
+      if (k <= 0) continue;
+      const int sp = s.med_sigma[m];
+      const int n[3] = {s.param_dims[4 * sp], s.param_dims[4 * sp + 1], s.param_dims[4 * sp + 2]};
+      for (int a = 0; a < 3; ++a) sg_res[m][a] = std::max(1, (n[a] + k - 1) / k);
+      build_supergrid(m, n, s.param_old[sp], sg_old[m]);
+      build_supergrid(m, n, s.param_new[sp], sg_new[m]);
+    }
+  }
+
+  // Cell maxima by brute force: voxel i (clamped to the grid) enters a cell
+  // spanning voxel coordinates [a, b] iff a trilinear lookup at some u in
+  // [a, b] reads it, i.e. iff [i - 1/2, i + 3/2) meets [a, b].
+  void build_supergrid(int m, const int n[3], const float *data, std::vector<R> &out) const {
+    const std::array<int, 3> &r = sg_res[m];
+    out.assign(size_t(r[0]) * r[1] * r[2], -std::numeric_limits<R>::infinity());
+    std::vector<std::vector<char>> in[3];
+    for (int a = 0; a < 3; ++a) {
+      in[a].assign(r[a], std::vector<char>(n[a], 0));
+      for (int c = 0; c < r[a]; ++c) {
+        const double lo = double(c) * n[a] / r[a], hi = double(c + 1) * n[a] / r[a];
+        for (int i = -2; i < n[a] + 2; ++i)
+          if (i - 0.5 <= hi && i + 1.5 > lo) in[a][c][std::min(std::max(i, 0), n[a] - 1)] = 1;
+      }
+    }
+    for (int cz = 0; cz < r[2]; ++cz)
+      for (int cy = 0; cy < r[1]; ++cy)
+        for (int cx = 0; cx < r[0]; ++cx) {
+          R mx = -std::numeric_limits<R>::infinity();
+          for (int z = 0; z < n[2]; ++z) {
+            if (!in[2][cz][z]) continue;
+            for (int y = 0; y < n[1]; ++y) {
+              if (!in[1][cy][y]) continue;
+              for (int x = 0; x < n[0]; ++x)
+                if (in[0][cx][x]) mx = std::max(mx, R(data[(size_t(z) * n[1] + y) * n[0] + x]));
+            }
+          }
+          out[(size_t(cz) * r[1] + cy) * r[0] + cx] = mx;
+        }
+  }
+
+  // Combined (max over both states) scaled majorant of the supergrid cell
+  // holding a cell-space point.
+  R sg_at(int m, const V<R> &q) const {
+    const std::array<int, 3> &r = sg_res[m];
+    int c[3];
+    for (int a = 0; a < 3; ++a)
+      c[a] = int(std::min(std::max(std::floor(q[a]), R(0)), R(r[a] - 1)));
+    const size_t i = (size_t(c[2]) * r[1] + c[1]) * r[0] + c[0];
+    return std::max(sg_new[m][i], sg_old[m][i]) * R(s.med_scale[m]);
+  }
+
+  // Free flight through the supergrid: 3-D DDA (Amanatides & Woo 1987) over
+  // the cells the ray crosses in [mint, maxt], accumulating the optical depth
+  // cell by cell until it reaches tau (plugins/twostatemedium.py:25-70).
+  // Returns +inf when the ray leaves first; `maj` is the majorant of the
+  // collision's cell (:82-85).
+  R dda_free_flight(int m, const Ray<R> &ray, R mint, R maxt, R tau, R &maj) const {
+    const R inf = std::numeric_limits<R>::infinity();
+    const float *lo = s.med_lo + 3 * m, *hi = s.med_hi + 3 * m;
+    R O[3], D[3], tm[3], td[3];
+    for (int a = 0; a < 3; ++a) {
+      const R k = R(sg_res[m][a]) / (R(hi[a]) - R(lo[a]));
+      O[a] = (ray.o[a] - R(lo[a])) * k;
+      D[a] = ray.d[a] * k;
+    }
+    for (int a = 0; a < 3; ++a) {
+      const R res = R(sg_res[m][a]);
+      // next face: upper face of the entry cell going up, lower face going down
+      const R cur = std::min(std::max(std::floor(std::fma(mint, D[a], O[a])), R(0)), res - R(1));
+      const R step = D[a] >= 0 ? R(1) : R(-1);
+      const R nb = D[a] >= 0 ? cur + R(1) : cur;
+      tm[a] = D[a] != 0 ? (nb - O[a]) / D[a] : inf;
+      td[a] = D[a] != 0 ? step / D[a] : inf;
+    }
+    R t = mint, tau_acc = 0;
+    bool reached = false, active = true;
+    const int guard = sg_res[m][0] + sg_res[m][1] + sg_res[m][2] + 4;
+    for (int it = 0; active && it < guard; ++it) {
+      const R t_next = std::min(std::min(tm[0], tm[1]), tm[2]);
+      const int axis = tm[0] == t_next ? 0 : (tm[1] == t_next ? 1 : 2);
+      const R th = R(0.5) * (t + t_next);
+      const R mj = sg_at(m, V<R>(std::fma(th, D[0], O[0]), std::fma(th, D[1], O[1]), std::fma(th, D[2], O[2])));
+      const R tau_next = tau_acc + mj * (t_next - t);
+      const R t_precise = t + (tau - tau_acc) / mj;
+      reached = mj > 0 && t_precise < maxt && tau_next >= tau;
+      t = reached ? t_precise : t_next;
+      active = !reached && t_next < maxt;
+      if (active) {
+        tm[axis] += td[axis];
+        tau_acc = tau_next;
+      }
+    }
+    const R ts = reached ? t : inf;
+    maj = ts <= maxt ? sg_at(m, V<R>(std::fma(ts, D[0], O[0]), std::fma(ts, D[1], O[1]), std::fma(ts, D[2], O[2])))
+                     : R(0);
+    return ts;
   }
 
   // ray_intersect: closest hit over all primitives, brute force
@@ -570,11 +677,18 @@ static MEI<R> sample_interaction_twostates(const Scene<R> &S, int med, const Ray
   R maxt = active ? std::min(ray.maxt, t1) : std::numeric_limits<R>::infinity();
   const int sp = S.s.med_sigma[med];
   const R scale = S.s.med_scale[med];
-  // global majorant max(maj_old, maj_new) (:71-75)
-  R combined = std::max(S.maj_old[sp] * scale, S.maj_new[sp] * scale);
-  active = active && combined > 0 && std::isfinite(combined);
   R desired_tau = -std::log(R(1) - sample);
-  R sampled_t = mint + desired_tau / combined;
+  R combined = 0, sampled_t = std::numeric_limits<R>::infinity();
+  if (S.sg_res[med][0] > 0) {
+    // majorant supergrid traversed by DDA (:25-70, 82-85)
+    if (active) sampled_t = S.dda_free_flight(med, ray, mint, maxt, desired_tau, combined);
+    active = active && combined > 0 && std::isfinite(combined);
+  } else {
+    // global majorant max(maj_old, maj_new) (:71-75)
+    combined = std::max(S.maj_old[sp] * scale, S.maj_new[sp] * scale);
+    active = active && combined > 0 && std::isfinite(combined);
+    sampled_t = mint + desired_tau / combined;
+  }
   bool valid_mi = active && (sampled_t <= maxt);
   mei.valid = valid_mi;
   mei.t = valid_mi ? sampled_t : std::numeric_limits<R>::infinity();
@@ -1179,6 +1293,20 @@ int oracle_adjoint(const OracleScene *s, const OracleIntegrator *I, uint32_t see
     oracle::adjoint<double>(*s, *I, seed, spp, grad_img, param, grad_out, n_threads);
   else
     oracle::adjoint<float>(*s, *I, seed, spp, grad_img, param, grad_out, n_threads);
+  return 0;
+}
+
+// Majorant supergrid of medium `med` for one state (0 old, 1 new): writes the
+// resolution to res[3] ((0, 0, 0) without a supergrid) and, when `out` is
+// not NULL, the unscaled cell maxima, x fastest.
+int oracle_majorant_grid(const OracleScene *s, int med, int state, float *out, int *res) {
+  if (!s || !res || med < 0 || med >= s->n_media) return -1;
+  oracle::Scene<double> S(*s);
+  for (int a = 0; a < 3; ++a) res[a] = S.sg_res[med][a];
+  if (out && res[0] > 0) {
+    const std::vector<double> &g = state == 0 ? S.sg_old[med] : S.sg_new[med];
+    for (size_t i = 0; i < g.size(); ++i) out[i] = float(g[i]);
+  }
   return 0;
 }
 

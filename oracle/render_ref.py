@@ -33,6 +33,7 @@ class OracleScene(C.Structure):
         ("med_hi", C.c_void_p), ("med_sample_emitters", C.c_void_p),
         ("n_params", C.c_int), ("param_dims", C.c_void_p), ("param_old", C.c_void_p), ("param_new", C.c_void_p),
         ("film_w", C.c_int), ("film_h", C.c_int), ("n_views", C.c_int), ("views", C.c_void_p),
+        ("med_majorant_factor", C.c_void_p),
     ]
 
 
@@ -58,6 +59,8 @@ def lib():
         _lib.oracle_adjoint.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleIntegrator), C.c_uint32, C.c_int,
                                         C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
         _lib.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
+        _lib.oracle_majorant_grid.restype = C.c_int
+        _lib.oracle_majorant_grid.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _lib.oracle_tea.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]
     return _lib
 
@@ -120,6 +123,7 @@ def make_scene(desc, params_old, params_new, sensor=0):
     s.med_lo = keep.arr([m.bbox_min for m in desc.media], np.float32)
     s.med_hi = keep.arr([m.bbox_max for m in desc.media], np.float32)
     s.med_sample_emitters = keep.arr([int(m.sample_emitters) for m in desc.media], np.int32)
+    s.med_majorant_factor = keep.arr([int(m.majorant_factor) for m in desc.media], np.int32)
     s.n_params = len(desc.params)
     dims = []
     for p in desc.params:
@@ -183,6 +187,21 @@ def adjoint(desc, max_depth, rr_depth, seed, spp, grad_img, param=0, params=None
         raise RuntimeError("oracle_adjoint failed")
     del keep
     return out[..., None]
+
+
+def majorant_grid(desc, params, medium=0, state=1):
+    """The oracle's unscaled majorant supergrid of a medium, shape (z, y, x),
+    or None when the medium uses one global majorant."""
+    s, keep = make_scene(desc, params, params, 0)
+    res = np.zeros(3, np.int32)
+    if lib().oracle_majorant_grid(C.byref(s), medium, state, None, res.ctypes.data) != 0:
+        raise RuntimeError("oracle_majorant_grid failed")
+    if res[0] == 0:
+        return None
+    out = np.zeros((res[2], res[1], res[0]), np.float32)
+    lib().oracle_majorant_grid(C.byref(s), medium, state, out.ctypes.data, res.ctypes.data)
+    del keep
+    return out
 
 
 def pcg32(initstate, initseq, n):

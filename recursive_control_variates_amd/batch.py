@@ -34,6 +34,8 @@ class BatchJob(C.Structure):
         ("clamp_lo", C.c_float), ("clamp_hi", C.c_float), ("img_ref", C.c_void_p), ("H", C.c_void_p),
         ("F", C.c_void_p), ("img", C.c_void_p), ("grad_img", C.c_void_p), ("cv_state", C.c_void_p * 8),
         ("warmup", C.c_int32), ("cv_beta1", C.c_float), ("cv_beta2", C.c_float), ("pre_update", C.c_int32),
+        ("resume", C.c_int32), ("iteration", C.c_int32), ("ring_old", C.c_int32), ("ring_new", C.c_int32),
+        ("ring_next", C.c_int32), ("next_key", C.c_uint32),
     ]
 
 
@@ -45,6 +47,9 @@ _BATCH_SIGS = {
     "rcv_batch_set_launch": [C.c_void_p, C.c_int32, C.c_int32],
     "rcv_batch_step": [C.c_void_p, C.c_int32, C.c_void_p],
     "rcv_batch_job_state": [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_void_p],
+    "rcv_batch_set_active": [C.c_void_p, C.POINTER(C.c_uint32), C.c_int32, C.c_void_p],
+    "rcv_batch_read_job": [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                           C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.c_void_p],
 }
 N.SIGNATURES.update(_BATCH_SIGS)
 
@@ -118,7 +123,12 @@ class _JobBuffers:
         sp.incoming = self.ring[1]
         sc._bind(sp)
 
-    def native(self) -> BatchJob:
+    def footprint_bytes(self) -> int:
+        """Device bytes this job holds (its buffers; the scene block is small)."""
+        ts = self.ring + [self.m, self.v, self.grad, self.img_ref, self.H, self.F, self.img, self.grad_img, self.cv]
+        return int(sum(t.numel() * t.element_size() for t in ts))
+
+    def native(self, resume=None) -> BatchJob:
         s = self.spec
         itype = _ITYPE[s.integrator["type"]]
         d = N.IntegratorDesc()
@@ -151,24 +161,33 @@ class _JobBuffers:
         j.warmup = s.warmup
         j.cv_beta1, j.cv_beta2 = s.cv_beta1, s.cv_beta2
         j.pre_update = int(s.pre_update)
+        if resume is not None:
+            j.resume = 1
+            j.iteration, j.ring_old, j.ring_new, j.ring_next, j.next_key = resume
         return j
 
 
 class BatchOptimizer:
-    def __init__(self, specs: List[JobSpec], device="cuda", use_graph=True, block_size=0, queue_grab=0):
+    """``specs``: JobSpec (new jobs) or _JobBuffers taken from another batch;
+    ``resume``: per job None or the tuple from ``read_job`` (state carried
+    over when a continuous-batching engine rebuilds its batch)."""
+
+    def __init__(self, specs: List, device="cuda", use_graph=True, block_size=0, queue_grab=0, resume=None,
+                 stream=None):
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.use_graph = bool(use_graph)
-        self.jobs = [_JobBuffers(s, self.device) for s in specs]
-        self.stream = torch.cuda.Stream(device=self.device)
+        self.jobs = [s if isinstance(s, _JobBuffers) else _JobBuffers(s, self.device) for s in specs]
+        resume = resume or [None] * len(self.jobs)
+        self.stream = stream if stream is not None else torch.cuda.Stream(device=self.device)
         self.stream.wait_stream(torch.cuda.current_stream(self.device))
         self._h = C.c_void_p()
         lib = _lib()
         N.check(lib.rcv_batch_create(C.byref(self._h)), "rcv_batch_create")
-        for jb in self.jobs:
+        for jb, rs in zip(self.jobs, resume):
             out = C.c_int32()
-            desc = jb.native()
+            desc = jb.native(rs)
             N.check(lib.rcv_batch_add_job(self._h, C.byref(desc), C.byref(out)), "rcv_batch_add_job")
         N.check(lib.rcv_batch_commit(self._h, C.c_void_p(self.stream.cuda_stream)), "rcv_batch_commit")
         self.set_launch(block_size, queue_grab)
@@ -194,6 +213,22 @@ class BatchOptimizer:
         N.check(_lib().rcv_batch_job_state(self._h, j, C.byref(it), C.byref(slot),
                                            C.c_void_p(self.stream.cuda_stream)), "rcv_batch_job_state")
         return it.value, slot.value
+
+    def read_job(self, j: int):
+        vals = [C.c_int32() for _ in range(4)] + [C.c_uint32()]
+        N.check(_lib().rcv_batch_read_job(self._h, j, *[C.byref(v) for v in vals],
+                                          C.c_void_p(self.stream.cuda_stream)), "rcv_batch_read_job")
+        return tuple(v.value for v in vals)
+
+    def set_active(self, mask):
+        """Continuous batching: only jobs with mask[j] true advance."""
+        n = len(self.jobs)
+        words = (C.c_uint32 * max(1, (n + 31) // 32))()
+        for j, on in enumerate(mask):
+            if on:
+                words[j >> 5] |= 1 << (j & 31)
+        N.check(_lib().rcv_batch_set_active(self._h, words, n, C.c_void_p(self.stream.cuda_stream)),
+                "rcv_batch_set_active")
 
     def param(self, j: int) -> torch.Tensor:
         """Latest optimiser value of job j's parameter (synchronises)."""

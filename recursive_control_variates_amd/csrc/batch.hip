@@ -9,6 +9,9 @@
 //               produced a `next` (the TwoState parameters_changed of
 //               plugins/twostatebsdf.py:46-56 / plugins/twostatemedium.py:
 //               189-200 as pointer rotation); reset per-iteration scratch
+//   1b. supergrid  majorant supergrids of both states of every optimised
+//               grid that has one (plugins/twostatemedium.py:25-70), rebuilt
+//               from the rotated ring
 //   2. render   persistent work-queue kernels over all jobs' (pixel, sample-
 //               chunk) units: two-state for cv_ps jobs, plain for baseline
 //               jobs (plugins/metaintegrator.py:68-88), surfaces and volumes
@@ -60,6 +63,7 @@ struct BatchJobDev {
   uint32_t *maj_keys;   // scene majorant keys of `param`: [OLD], [NEW]
   uint32_t next_key;    // max key of the ring's next buffer (written by adam)
   int32_t iter;
+  int32_t active;       // 0: the job sits out this iteration (no state changes)
   float *H, *F, *img, *grad_img;
   const float *img_ref;
   float *cv[8];
@@ -71,6 +75,8 @@ struct BatchJobDev {
   GradTargets gt;       // gt.g[param] = grad, others null
   float lr, b1, b2, eps, clamp_lo, clamp_hi;
   double log_b1, log_b2;
+  float *sg_out;        // supergrids [OLD][NEW] of the optimised grid's medium, or null
+  int32_t sg_res[3];
 };
 
 namespace {
@@ -106,6 +112,7 @@ __global__ void batch_rotate_kernel(BatchJobDev *jobs, int n_jobs, unsigned *que
   if (j < 6) queues[j] = 0u;
   if (j >= n_jobs) return;
   BatchJobDev &J = jobs[j];
+  if (!J.active) return;
   if (J.iter > 0) {
     const int old_slot = J.ring_old;
     J.ring_old = J.ring_new;
@@ -117,6 +124,25 @@ __global__ void batch_rotate_kernel(BatchJobDev *jobs, int n_jobs, unsigned *que
     J.maj_keys[RCV_STATE_NEW] = J.next_key;
   }
   J.next_key = float_key_dev(-kInf);
+}
+
+// ----------------------------------------------------------- 1b. supergrid
+// One thread per (job, state, cell) over the jobs whose optimised grid has a
+// majorant supergrid; ends[] are the cumulative 2 * cells counts.
+__global__ void batch_supergrid_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
+                                       const int64_t *__restrict__ ends, int n, int64_t total) {
+  const int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= total) return;
+  const int k = find_job(ends, n, u);
+  const BatchJobDev &J = jobs[ids[k]];
+  if (!J.active) return;
+  const int cells = J.sg_res[0] * J.sg_res[1] * J.sg_res[2];
+  const int local = int(u - (k > 0 ? ends[k - 1] : 0));
+  const int state = local >= cells ? RCV_STATE_NEW : RCV_STATE_OLD;
+  const int c = local - (state == RCV_STATE_NEW ? cells : 0);
+  const ParamBind &P = J.view.params[J.param];
+  J.sg_out[state * cells + c] =
+      supergrid_cell_max(P.data[state], P.nx, P.ny, P.nz, J.sg_res[0], J.sg_res[1], J.sg_res[2], c);
 }
 
 // ---------------------------------------------------------------- 2. render
@@ -136,6 +162,7 @@ __global__ __launch_bounds__(512) void batch_render_kernel(const BatchJobDev *__
     for (int64_t u0 = base; u0 < stop; u0 += kWave) {
       const int k = __builtin_amdgcn_readfirstlane(find_job(ends, n, u0));
       const BatchJobDev &J = jobs[ids[k]];
+      if (!J.active) continue;
       const SceneView &S = J.view;
       const int64_t local = u0 + lane - J.unit_begin;
       const int64_t pixel = local >> J.cl2;
@@ -196,6 +223,7 @@ __global__ __launch_bounds__(kElemBlock) void batch_combine_loss_kernel(BatchJob
   const int64_t e0 = int64_t(blockIdx.x) * kElemBlock;
   const int j = __builtin_amdgcn_readfirstlane(find_job(ends, n_jobs, e0));
   const BatchJobDev &J = jobs[j];
+  if (!J.active) return;
   const int64_t i = e0 + threadIdx.x - J.elem_begin;
   if (i >= J.n_img) return;
   const float f = J.F[i];
@@ -261,6 +289,7 @@ __global__ __launch_bounds__(kElemBlock) void batch_zero_grad_kernel(BatchJobDev
   const int64_t e0 = int64_t(blockIdx.x) * kElemBlock;
   const int j = __builtin_amdgcn_readfirstlane(find_job(ends, n_jobs, e0));
   const BatchJobDev &J = jobs[j];
+  if (!J.active) return;
   const int64_t i = e0 + threadIdx.x - J.pelem_begin;
   if (i < J.n_param) J.grad[i] = 0.0f;
 }
@@ -281,6 +310,7 @@ __global__ __launch_bounds__(512) void batch_adjoint_kernel(const BatchJobDev *_
     for (int64_t u0 = base; u0 < stop; u0 += kWave) {
       const int k = __builtin_amdgcn_readfirstlane(find_job(ends, n, u0));
       const BatchJobDev &J = jobs[ids[k]];
+      if (!J.active) continue;
       const SceneView &S = J.view;
       const int64_t local = u0 + lane - J.unit_begin_g;
       const int64_t pixel = local >> J.cl2_g;
@@ -327,6 +357,7 @@ __global__ __launch_bounds__(kElemBlock) void batch_adam_kernel(BatchJobDev *__r
   const int64_t e0 = int64_t(blockIdx.x) * kElemBlock;
   const int j = __builtin_amdgcn_readfirstlane(find_job(ends, n_jobs, e0));
   BatchJobDev &J = jobs[j];
+  if (!J.active) return;
   const int64_t i = e0 + threadIdx.x - J.pelem_begin;
   float out = -kInf;
   if (i < J.n_param) {
@@ -353,7 +384,7 @@ __global__ __launch_bounds__(kElemBlock) void batch_adam_kernel(BatchJobDev *__r
 // ---------------------------------------------------------- 7. advance
 __global__ void batch_advance_kernel(BatchJobDev *jobs, int n_jobs) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n_jobs) jobs[j].iter += 1;
+  if (j < n_jobs && jobs[j].active) jobs[j].iter += 1;
 }
 
 }  // namespace
@@ -379,6 +410,10 @@ struct Batch {
   BatchClass adj_cls[2];     // itype
   int32_t *d_ids[6] = {nullptr};
   int64_t *d_ends[6] = {nullptr};
+  std::vector<int32_t> sg_ids;   // jobs whose optimised grid has a supergrid
+  std::vector<int64_t> sg_ends;  // cumulative 2 * cells
+  int32_t *d_sg_ids = nullptr;
+  int64_t *d_sg_ends = nullptr;
   int block = 256, grab = 256, grid_render = 0, grid_adj = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -448,13 +483,19 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
   J.n_param = int64_t(P.nx) * P.ny * P.nz * P.ch;
   J.n_img = int64_t(J.n_pix) * 3;
   for (int k = 0; k < 3; ++k) J.ring[k] = d->param_ring[k];
-  J.ring_old = 0;
-  J.ring_new = 1;
-  J.ring_next = 2;
-  J.view.params[d->param].data[RCV_STATE_OLD] = J.ring[0];
-  J.view.params[d->param].data[RCV_STATE_NEW] = J.ring[1];
+  J.ring_old = d->resume ? d->ring_old : 0;
+  J.ring_new = d->resume ? d->ring_new : 1;
+  J.ring_next = d->resume ? d->ring_next : 2;
+  RCV_REQUIRE(J.ring_old != J.ring_new && J.ring_new != J.ring_next && J.ring_old != J.ring_next &&
+                  J.ring_old >= 0 && J.ring_old < 3 && J.ring_new >= 0 && J.ring_new < 3 && J.ring_next >= 0 &&
+                  J.ring_next < 3,
+              "rcv_batch_add_job: ring slots must be a permutation of 0, 1, 2");
+  J.view.params[d->param].data[RCV_STATE_OLD] = J.ring[J.ring_old];
+  J.view.params[d->param].data[RCV_STATE_NEW] = J.ring[J.ring_new];
   J.maj_keys = s->majorant_keys + 2 * d->param;
-  J.iter = 0;
+  J.iter = d->resume ? d->iteration : 0;
+  J.active = 1;
+  J.next_key = d->resume ? d->next_key : 0u;
   J.H = d->H;
   J.F = d->F;
   J.img = d->img;
@@ -480,6 +521,15 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
   J.clamp_hi = d->clamp_hi;
   J.log_b1 = log(double(d->beta1));
   J.log_b2 = log(double(d->beta2));
+  J.sg_out = nullptr;
+  int n_sg = 0;
+  for (const MediumInfo &m : s->media)
+    if (m.sigma_param == d->param && m.supergrid_offset >= 0) {
+      ++n_sg;
+      J.sg_out = s->supergrids + m.supergrid_offset;
+      for (int k = 0; k < 3; ++k) J.sg_res[k] = m.sg_res[k];
+    }
+  RCV_REQUIRE(n_sg <= 1, "rcv_batch_add_job: the optimised grid drives %d media with supergrids (at most 1)", n_sg);
   b->host_jobs.push_back(J);
   b->scenes.push_back(s);
   if (out_job) *out_job = int(b->host_jobs.size()) - 1;
@@ -492,7 +542,7 @@ extern "C" int rcv_batch_commit(rcv_batch *b, void *stream) {
   RCV_REQUIRE(b, "rcv_batch_commit: null batch");
   if (b->committed) return fail(RCV_E_STATE, "rcv_batch_commit: already committed");
   const int n = int(b->host_jobs.size());
-  RCV_REQUIRE(n > 0, "rcv_batch_commit: no jobs");
+  RCV_REQUIRE(n > 0 && n <= RCV_BATCH_MAX_JOBS, "rcv_batch_commit: 1..%d jobs", RCV_BATCH_MAX_JOBS);
   std::vector<int64_t> elem_ends(n), pelem_ends(n);
   int64_t e = 0, pe = 0;
   for (int j = 0; j < n; ++j) {
@@ -521,6 +571,16 @@ extern "C" int rcv_batch_commit(rcv_batch *b, void *stream) {
     RCV_REQUIRE(c.total < (int64_t(1) << 31), "rcv_batch_commit: too many work units");
   b->elem_total = e;
   b->pelem_total = pe;
+  b->sg_ids.clear();
+  b->sg_ends.clear();
+  int64_t sg_total = 0;
+  for (int j = 0; j < n; ++j) {
+    const BatchJobDev &J = b->host_jobs[j];
+    if (!J.sg_out) continue;
+    sg_total += 2 * int64_t(J.sg_res[0]) * J.sg_res[1] * J.sg_res[2];
+    b->sg_ids.push_back(j);
+    b->sg_ends.push_back(sg_total);
+  }
   // device block: jobs | elem ends | pelem ends | queues | class ids/ends
   size_t bytes = 0;
   auto reserve = [&](size_t sz) {
@@ -532,6 +592,8 @@ extern "C" int rcv_batch_commit(rcv_batch *b, void *stream) {
   const size_t o_ee = reserve(8 * n);
   const size_t o_pe = reserve(8 * n);
   const size_t o_q = reserve(64);
+  const size_t o_sgi = reserve(4 * b->sg_ids.size());
+  const size_t o_sge = reserve(8 * b->sg_ends.size());
   size_t o_ids[6], o_ends[6];
   for (int c = 0; c < 6; ++c) {
     const BatchClass &bc = c < 4 ? b->render_cls[c] : b->adj_cls[c - 4];
@@ -544,6 +606,12 @@ extern "C" int rcv_batch_commit(rcv_batch *b, void *stream) {
   std::memcpy(blob.data() + o_jobs, b->host_jobs.data(), sizeof(BatchJobDev) * n);
   std::memcpy(blob.data() + o_ee, elem_ends.data(), 8 * n);
   std::memcpy(blob.data() + o_pe, pelem_ends.data(), 8 * n);
+  if (!b->sg_ids.empty()) {
+    std::memcpy(blob.data() + o_sgi, b->sg_ids.data(), 4 * b->sg_ids.size());
+    std::memcpy(blob.data() + o_sge, b->sg_ends.data(), 8 * b->sg_ends.size());
+  }
+  b->d_sg_ids = reinterpret_cast<int32_t *>(base + o_sgi);
+  b->d_sg_ends = reinterpret_cast<int64_t *>(base + o_sge);
   for (int c = 0; c < 6; ++c) {
     const BatchClass &bc = c < 4 ? b->render_cls[c] : b->adj_cls[c - 4];
     if (!bc.ids.empty()) {
@@ -607,6 +675,11 @@ static int enqueue_step(rcv_batch *b, hipStream_t st) {
   const size_t lds = size_t(kStackSize) * b->block * sizeof(int);
   hipLaunchKernelGGL(batch_rotate_kernel, dim3(ceil_div(std::max(n, 6), 64)), dim3(64), 0, st, b->d_jobs, n,
                      b->d_queues);
+  if (!b->sg_ids.empty()) {
+    const int64_t total = b->sg_ends.back();
+    hipLaunchKernelGGL(batch_supergrid_kernel, dim3(ceil_div(total, 128)), dim3(128), 0, st, b->d_jobs,
+                       b->d_sg_ids, b->d_sg_ends, int(b->sg_ids.size()), total);
+  }
   // render classes: (path, 1), (path, 2), (vol, 1), (vol, 2); queues 0..3 reused per class in sequence
   for (int c = 0; c < 4; ++c) {
     const BatchClass &bc = b->render_cls[c];
@@ -670,6 +743,46 @@ extern "C" int rcv_batch_step(rcv_batch *b, int32_t use_graph, void *stream) {
     RCV_TRY_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "rcv_batch_step: instantiate");
   }
   RCV_TRY_HIP(hipGraphLaunch(b->graph_exec, st), "rcv_batch_step: graph launch");
+  return RCV_OK;
+}
+
+namespace rcv {
+namespace {
+struct ActiveBits {
+  uint32_t w[RCV_BATCH_MAX_JOBS / 32];
+};
+__global__ void batch_set_active_kernel(BatchJobDev *jobs, int n_jobs, ActiveBits bits) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n_jobs) jobs[j].active = (bits.w[j >> 5] >> (j & 31)) & 1u;
+}
+}  // namespace
+}  // namespace rcv
+
+extern "C" int rcv_batch_set_active(rcv_batch *b, const uint32_t *active_bits, int32_t n_jobs, void *stream) {
+  RCV_REQUIRE(b && b->committed, "rcv_batch_set_active: batch not committed");
+  RCV_REQUIRE(active_bits, "rcv_batch_set_active: null flags");
+  RCV_REQUIRE(n_jobs == int(b->host_jobs.size()), "rcv_batch_set_active: expected %d jobs, got %d",
+              int(b->host_jobs.size()), n_jobs);
+  ActiveBits bits{};
+  for (int w = 0; w < (n_jobs + 31) / 32; ++w) bits.w[w] = active_bits[w];
+  hipLaunchKernelGGL(batch_set_active_kernel, dim3(ceil_div(n_jobs, 64)), dim3(64), 0, as_stream(stream),
+                     b->d_jobs, n_jobs, bits);
+  return check_launch("rcv_batch_set_active");
+}
+
+extern "C" int rcv_batch_read_job(const rcv_batch *b, int32_t job, int32_t *iteration, int32_t *ring_old,
+                                  int32_t *ring_new, int32_t *ring_next, uint32_t *next_key, void *stream) {
+  RCV_REQUIRE(b && b->committed, "rcv_batch_read_job: batch not committed");
+  RCV_REQUIRE(job >= 0 && job < int(b->host_jobs.size()), "rcv_batch_read_job: bad job");
+  BatchJobDev J;
+  hipStream_t st = as_stream(stream);
+  RCV_TRY_HIP(hipMemcpyAsync(&J, b->d_jobs + job, sizeof(J), hipMemcpyDeviceToHost, st), "rcv_batch_read_job");
+  RCV_TRY_HIP(hipStreamSynchronize(st), "rcv_batch_read_job: synchronize");
+  if (iteration) *iteration = J.iter;
+  if (ring_old) *ring_old = J.ring_old;
+  if (ring_new) *ring_new = J.ring_new;
+  if (ring_next) *ring_next = J.ring_next;
+  if (next_key) *next_key = J.next_key;
   return RCV_OK;
 }
 

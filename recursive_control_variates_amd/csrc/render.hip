@@ -251,8 +251,23 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   return check_launch(fn);
 }
 
-int Scene::build_supergrid(int, int, hipStream_t) {
-  return fail(RCV_E_INVALID, "majorant supergrid: not supported by this build");
+namespace {
+__global__ void supergrid_kernel(const float *__restrict__ grid, int nx, int ny, int nz, int sx, int sy, int sz,
+                                 float *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= sx * sy * sz) return;
+  out[c] = supergrid_cell_max(grid, nx, ny, nz, sx, sy, sz, c);
+}
+}  // namespace
+
+int Scene::build_supergrid(int medium, int state, hipStream_t stream) {
+  const MediumInfo &m = media[medium];
+  const Param &p = params[m.sigma_param];
+  const int cells = m.sg_res[0] * m.sg_res[1] * m.sg_res[2];
+  float *out = supergrids + m.supergrid_offset + state * cells;
+  hipLaunchKernelGGL(supergrid_kernel, dim3(ceil_div(cells, 128)), dim3(128), 0, stream, p.data[state], p.nx, p.ny,
+                     p.nz, m.sg_res[0], m.sg_res[1], m.sg_res[2], out);
+  return check_launch("majorant supergrid");
 }
 
 }  // namespace rcv

@@ -640,6 +640,29 @@ extern "C" int rcv_scene_set_counting(rcv_scene *s, int32_t enabled) {
   return RCV_OK;
 }
 
+extern "C" int rcv_scene_majorant_grid(const rcv_scene *s, int32_t medium, int32_t state, float *out,
+                                       int64_t capacity, int32_t *res, void *stream) {
+  RCV_REQUIRE(s && s->committed, "rcv_scene_majorant_grid: scene not committed");
+  RCV_REQUIRE(res, "rcv_scene_majorant_grid: null res");
+  RCV_REQUIRE(medium >= 0 && medium < int(s->media.size()), "rcv_scene_majorant_grid: bad medium %d", medium);
+  RCV_REQUIRE(state == RCV_STATE_OLD || state == RCV_STATE_NEW, "rcv_scene_majorant_grid: bad state");
+  const MediumInfo &m = s->media[medium];
+  if (m.supergrid_offset < 0) {
+    res[0] = res[1] = res[2] = 0;
+    return RCV_OK;
+  }
+  const int64_t cells = int64_t(m.sg_res[0]) * m.sg_res[1] * m.sg_res[2];
+  for (int k = 0; k < 3; ++k) res[k] = m.sg_res[k];
+  if (!out) return RCV_OK;
+  RCV_REQUIRE(capacity >= cells, "rcv_scene_majorant_grid: capacity %lld < %lld cells", (long long)capacity,
+              (long long)cells);
+  RCV_TRY_HIP(hipMemcpyAsync(out, s->supergrids + m.supergrid_offset + state * cells, cells * sizeof(float),
+                             hipMemcpyDeviceToHost, as_stream(stream)),
+              "rcv_scene_majorant_grid");
+  RCV_TRY_HIP(hipStreamSynchronize(as_stream(stream)), "rcv_scene_majorant_grid: synchronize");
+  return RCV_OK;
+}
+
 extern "C" int rcv_scene_read_counters(const rcv_scene *s, int64_t *out, int32_t n, void *stream) {
   RCV_REQUIRE(s && s->committed, "rcv_scene_read_counters: scene not committed");
   RCV_REQUIRE(out && n > 0 && n <= kNumCounters, "rcv_scene_read_counters: 1..%d counters", kNumCounters);

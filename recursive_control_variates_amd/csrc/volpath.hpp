@@ -83,8 +83,112 @@ __device__ inline bool ray_aabb(const Ray &r, const float lo[3], const float hi[
   return t0 <= t1;
 }
 
+__device__ inline int floor_div_i(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// Voxel range [i0, i1] that trilinear lookups anywhere in supergrid cell c of
+// s cells along an axis of n voxels read from: the cell spans voxel
+// coordinates [c n / s, (c + 1) n / s] and a lookup at u reads voxels
+// floor(u - 1/2) and floor(u - 1/2) + 1 (clamped to the grid).
+__device__ inline void supergrid_cell_voxels(int c, int n, int s, int &i0, int &i1) {
+  i0 = max(floor_div_i(2 * c * n - s, 2 * s), 0);
+  i1 = min(floor_div_i(2 * (c + 1) * n - s, 2 * s) + 1, n - 1);
+}
+
+// Majorant of supergrid cell `c` (x fastest) of an (sx, sy, sz) supergrid
+// over an (nx, ny, nz) grid: the maximum over every voxel a point of the cell
+// can interpolate from, so it bounds the interpolated grid in the cell.
+__device__ inline float supergrid_cell_max(const float *__restrict__ grid, int nx, int ny, int nz, int sx, int sy,
+                                           int sz, int c) {
+  const int cx = c % sx, cy = (c / sx) % sy, cz = c / (sx * sy);
+  int x0, x1, y0, y1, z0, z1;
+  supergrid_cell_voxels(cx, nx, sx, x0, x1);
+  supergrid_cell_voxels(cy, ny, sy, y0, y1);
+  supergrid_cell_voxels(cz, nz, sz, z0, z1);
+  float m = -kInf;
+  for (int z = z0; z <= z1; ++z)
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) m = fmaxf(m, grid[(int64_t(z) * ny + y) * nx + x]);
+  return m;
+}
+
+// Supergrid cell of a point given in cell space (unit cells), clamped to the
+// grid; the product of the cell's majorant over the states in flight and the
+// medium scale. NS == 1 reads the NEW state only.
+template <int NS>
+__device__ inline float supergrid_at(const SceneView &S, const MediumInfo &M, V3 q) {
+  const int sx = M.sg_res[0], sy = M.sg_res[1], sz = M.sg_res[2];
+  const int cx = int(fminf(fmaxf(floorf(q.x), 0.0f), float(sx - 1)));
+  const int cy = int(fminf(fmaxf(floorf(q.y), 0.0f), float(sy - 1)));
+  const int cz = int(fminf(fmaxf(floorf(q.z), 0.0f), float(sz - 1)));
+  const int cells = sx * sy * sz;
+  const float *g = S.supergrids + M.supergrid_offset + (cz * sy + cy) * sx + cx;
+  float m = g[RCV_STATE_NEW * cells];
+  if (NS == 2) m = fmaxf(m, g[RCV_STATE_OLD * cells]);
+  return m * M.scale;
+}
+
+// Free-flight distance through a piecewise-constant majorant: a 3-D DDA
+// (Amanatides & Woo 1987, "A Fast Voxel Traversal Algorithm") over the
+// supergrid cells the ray crosses in [mint, maxt], accumulating optical depth
+// until it reaches tau (plugins/twostatemedium.py:25-70). The ray is carried
+// in cell space (each supergrid cell a unit cube), where its parameter t is
+// the world ray's. Returns kInf when the ray leaves the medium first; sets
+// `maj` to the majorant of the cell the collision lies in (:82-85).
+template <int NS>
+__device__ inline float supergrid_free_flight(const SceneView &S, const MediumInfo &M, const Ray &ray, float mint,
+                                              float maxt, float tau, float &maj) {
+  const float kx = float(M.sg_res[0]) / (M.hi[0] - M.lo[0]);
+  const float ky = float(M.sg_res[1]) / (M.hi[1] - M.lo[1]);
+  const float kz = float(M.sg_res[2]) / (M.hi[2] - M.lo[2]);
+  const V3 O = v3((ray.o.x - M.lo[0]) * kx, (ray.o.y - M.lo[1]) * ky, (ray.o.z - M.lo[2]) * kz);
+  const V3 D = v3(ray.d.x * kx, ray.d.y * ky, ray.d.z * kz);
+  // Per axis: the parameter of the next cell face the ray meets (the upper
+  // face of the entry cell for an increasing coordinate, its lower face for a
+  // decreasing one) and the parameter step between faces. Every segment
+  // [t, min(next faces)] then lies in one cell, so the majorant looked up at
+  // its midpoint is the cell's.
+  float tm[3], td[3];
+  const float o3[3] = {O.x, O.y, O.z}, d3[3] = {D.x, D.y, D.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float res = float(M.sg_res[k]);
+    const float cur = fminf(fmaxf(floorf(fmaf(mint, d3[k], o3[k])), 0.0f), res - 1.0f);
+    const float step = d3[k] >= 0.0f ? 1.0f : -1.0f;
+    const float nb = d3[k] >= 0.0f ? cur + 1.0f : cur;
+    const bool moving = d3[k] != 0.0f;
+    tm[k] = moving ? (nb - o3[k]) / d3[k] : kInf;
+    td[k] = moving ? step / d3[k] : kInf;
+  }
+  float t = mint, tau_acc = 0.0f;
+  bool reached = false, active = true;
+  // A ray crosses at most sx + sy + sz faces inside the grid; the bound only
+  // guards against non-finite input.
+  const int guard = M.sg_res[0] + M.sg_res[1] + M.sg_res[2] + 4;
+  for (int it = 0; active && it < guard; ++it) {
+    const float t_next = fminf(fminf(tm[0], tm[1]), tm[2]);
+    const bool a0 = tm[0] == t_next, a1 = !a0 && tm[1] == t_next, a2 = !a0 && !a1;
+    const float th = 0.5f * (t + t_next);
+    const float m = supergrid_at<NS>(S, M, v3(fmaf(th, D.x, O.x), fmaf(th, D.y, O.y), fmaf(th, D.z, O.z)));
+    const float tau_next = tau_acc + m * (t_next - t);
+    const float t_precise = t + (tau - tau_acc) / m;
+    reached = m > 0.0f && t_precise < maxt && tau_next >= tau;
+    t = reached ? t_precise : t_next;
+    active = !reached && t_next < maxt;
+    if (active) {
+      tm[0] += a0 ? td[0] : 0.0f;
+      tm[1] += a1 ? td[1] : 0.0f;
+      tm[2] += a2 ? td[2] : 0.0f;
+      tau_acc = tau_next;
+    }
+  }
+  const float ts = reached ? t : kInf;
+  maj = ts <= maxt ? supergrid_at<NS>(S, M, v3(fmaf(ts, D.x, O.x), fmaf(ts, D.y, O.y), fmaf(ts, D.z, O.z))) : 0.0f;
+  return ts;
+}
+
 // TwoStateMedium.sample_interaction_twostates (plugins/twostatemedium.py:
-// 15-107) with one global majorant max(maj_old, maj_new) (:71-78).
+// 15-107): one global majorant max(maj_old, maj_new) (:71-78), or the
+// supergrid DDA when the medium has a majorant supergrid (:25-70, 82-85).
 template <int NS, bool FOOT>
 __device__ inline void sample_medium(const SceneView &S, int mid, const Ray &ray, float u, MediumEv<NS> &ev) {
   const MediumInfo &M = S.media[mid];
@@ -94,15 +198,23 @@ __device__ inline void sample_medium(const SceneView &S, int mid, const Ray &ray
   const float mint = ok ? fmaxf(0.0f, t0) : 0.0f;
   const float maxt = ok ? fminf(ray.maxt, t1) : kInf;
   const float scale = M.scale;
-  float maj = majorant_of(S, M.sigma_param, RCV_STATE_NEW);
-  if (NS == 2) maj = fmaxf(maj, majorant_of(S, M.sigma_param, RCV_STATE_OLD));
-  maj *= scale;
-  // A medium whose majorant is not a positive finite number has no
-  // collisions (t = inf for a zero majorant, as in the reference); this also
-  // keeps every sampled step strictly forward.
-  ok = ok && maj > 0.0f && maj < kInf;
   const float tau = -logf(1.0f - u);
-  const float t = mint + tau / maj;
+  float maj = 0.0f, t = kInf;
+  if (M.supergrid_offset >= 0) {
+    if (ok) t = supergrid_free_flight<NS>(S, M, ray, mint, maxt, tau, maj);
+    // A collision is kept only where its cell's majorant is positive and
+    // finite, the invariant of the single-majorant branch below.
+    ok = ok && maj > 0.0f && maj < kInf;
+  } else {
+    maj = majorant_of(S, M.sigma_param, RCV_STATE_NEW);
+    if (NS == 2) maj = fmaxf(maj, majorant_of(S, M.sigma_param, RCV_STATE_OLD));
+    maj *= scale;
+    // A medium whose majorant is not a positive finite number has no
+    // collisions (t = inf for a zero majorant, as in the reference); this
+    // also keeps every sampled step strictly forward.
+    ok = ok && maj > 0.0f && maj < kInf;
+    t = mint + tau / maj;
+  }
   ev.valid = ok && (t <= maxt);
   ev.mint = mint;
   ev.maj = maj;

@@ -190,6 +190,21 @@ class Scene:
         N.call("rcv_scene_read_counters", self._handle, out, 8, C.c_void_p(N.stream_handle(self.device)))
         return dict(zip(N.COUNTER_NAMES, [int(x) for x in out]))
 
+    def majorant_grid(self, medium: int = 0, state: int = 1):
+        """Unscaled majorant supergrid of a medium for one state (0 old,
+        1 new) as a float32 array of shape (z, y, x), or None when the medium
+        uses one global majorant (Medium.majorant_grid(), read at
+        plugins/twostatemedium.py:25-29)."""
+        res = (C.c_int32 * 3)()
+        stream = C.c_void_p(N.stream_handle(self.device))
+        N.call("rcv_scene_majorant_grid", self._handle, medium, state, None, 0, res, stream)
+        if res[0] == 0:
+            return None
+        out = np.empty((res[2], res[1], res[0]), np.float32)
+        N.call("rcv_scene_majorant_grid", self._handle, medium, state, out.ctypes.data_as(C.c_void_p), out.size,
+               res, stream)
+        return out
+
     def _destroy(self):
         if getattr(self, "_handle", None) is not None and self._handle.value:
             try:

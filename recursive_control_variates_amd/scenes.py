@@ -122,3 +122,49 @@ def cornell_scene_dict(texture: np.ndarray, twostate: bool = True, film_res: int
 VOLUME_KEY = "object.interior_medium.sigma_t.data"
 ALBEDO_KEY = "object.interior_medium.albedo.data"
 TEXTURE_KEY = "back.bsdf.reflectance.data"
+
+
+# ------------------------------------------------------------ job mixes
+def _side(rng):
+    return int(rng.integers(2, 17)) * 32          # U{64, 96, ..., 512}
+
+
+def volume_job_params(seed: int):
+    """Size parameters of one volume job of the batched / served mix: single
+    view of side^2 (side U{64..512} step 32), grid side U{32, 64, 128}, spp
+    U{4..64}, max_depth U{8..32}, drawn from default_rng(seed + 7919)."""
+    rng = np.random.default_rng(seed + 7919)
+    side = _side(rng)
+    res = int(rng.choice([32, 64, 128]))
+    spp = int(rng.integers(4, 65))
+    depth = int(rng.integers(8, 33))
+    return dict(kind="volume", side=side, grid=res, spp=spp, max_depth=depth, key=VOLUME_KEY)
+
+
+def surface_job_params(seed: int):
+    """Size parameters of one surface job: side^2 film, spp U{4..64},
+    max_depth 4, drawn from default_rng(seed + 104729)."""
+    rng = np.random.default_rng(seed + 104729)
+    side = _side(rng)
+    spp = int(rng.integers(4, 65))
+    return dict(kind="surface", side=side, grid=64, spp=spp, max_depth=4, key=TEXTURE_KEY)
+
+
+def volume_job_dict(seed: int, twostate=True):
+    """Scene and description of one volume job (blob target from ``seed``,
+    sigma_t scale 20, albedo 0.8, initial grid 0.1)."""
+    m = volume_job_params(seed)
+    m["target"] = blob_grid(m["grid"], seed)
+    m["init"] = np.full_like(m["target"], 0.1)
+    d = volume_scene_dict(m["init"], twostate=twostate, film_res=m["side"], n_views=1, spp=m["spp"])
+    return d, m
+
+
+def surface_job_dict(seed: int, twostate=True):
+    """Scene and description of one surface job: the textured Cornell box with
+    a 64x64 texture target from ``seed``, initial texture 0.5."""
+    m = surface_job_params(seed)
+    m["target"] = texture_target(64, seed)
+    m["init"] = np.full_like(m["target"], 0.5)
+    d = cornell_scene_dict(m["init"], twostate=twostate, film_res=m["side"], spp=m["spp"])
+    return d, m

@@ -11,20 +11,21 @@ def cam(origin, target, fov=45.0, w=8, h=8, spp=4, up=(0, 1, 0)):
             "sampler": {"type": "independent", "sample_count": spp}}
 
 
-def medium_dict(grid, albedo, scale, twostate=False):
+def medium_dict(grid, albedo, scale, twostate=False, majorant_factor=0):
     m = {"type": "heterogeneous", "sigma_t": {"type": "gridvolume", "data": grid, "to_world": T.scale(1.0)},
-         "albedo": albedo, "scale": scale}
+         "albedo": albedo, "scale": scale, "majorant_resolution_factor": majorant_factor}
     return {"type": "twostatemedium", "old": m, "new": m, "incoming": m} if twostate else m
 
 
 def volume_box_scene(grid, albedo=0.8, scale=5.0, env=1.0, w=8, h=8, spp=4, twostate=True, fov=40.0,
-                     origin=(0.5, 0.5, 2.6)):
+                     origin=(0.5, 0.5, 2.6), majorant_factor=0):
     """Grid medium in a null unit cube [0,1]^3 under a constant environment,
-    one camera on +z looking at the cube centre."""
+    one camera on +z looking at the cube centre (``majorant_factor`` > 0:
+    majorant supergrid of ceil(n / factor) cells per axis)."""
     return {
         "type": "scene",
         "box": {"type": "cube", "bsdf": {"type": "null"}, "to_world": T.translate([0.5, 0.5, 0.5]).scale(0.5),
-                "interior": medium_dict(grid, albedo, scale, twostate)},
+                "interior": medium_dict(grid, albedo, scale, twostate, majorant_factor)},
         "env": {"type": "constant", "radiance": {"type": "rgb", "value": env}},
         "sensor": cam(origin, (0.5, 0.5, 0.5), fov=fov, w=w, h=h, spp=spp),
     }

@@ -15,10 +15,11 @@ pytestmark = pytest.mark.gpu
 KEY = "box.interior_medium.sigma_t.data"
 
 
-def _setup(seed=0, n=5, w=8, h=8, scale=5.0):
+def _setup(seed=0, n=5, w=8, h=8, scale=5.0, majorant_factor=0):
     rng = np.random.default_rng(seed)
     grid = (0.2 + rng.uniform(0, 0.8, (n, n, n, 1))).astype(np.float32)
-    sc = rcv.load_dict(volume_box_scene(grid, albedo=0.8, scale=scale, w=w, h=h, spp=16))
+    sc = rcv.load_dict(volume_box_scene(grid, albedo=0.8, scale=scale, w=w, h=h, spp=16,
+                                        majorant_factor=majorant_factor))
     return rng, grid, sc
 
 
@@ -32,9 +33,11 @@ def _adjoint(sc, integ, gimg, seed, spp):
     return grads[KEY].cpu().numpy()
 
 
-def test_adjoint_matches_oracle_same_samples(cuda):
-    announce("adjoint vs oracle fp32")
-    rng, grid, sc = _setup(1)
+@pytest.mark.parametrize("factor", [0, 2])
+def test_adjoint_matches_oracle_same_samples(cuda, factor):
+    # factor 2: free flights and NEE ratio tracking through a 3^3 supergrid
+    announce(f"adjoint vs oracle fp32 (majorant factor {factor})")
+    rng, grid, sc = _setup(1, majorant_factor=factor)
     gimg = rng.uniform(0.5, 1.5, (8, 8, 3))
     integ = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
     g_gpu = _adjoint(sc, integ, gimg, 31, 32)
@@ -117,6 +120,42 @@ def test_texture_adjoint_matches_finite_differences(cuda):
         p.update()
         p.update()
         g = plain.adjoint(sc, p, torch.from_numpy(gimg.astype(np.float32)).cuda(), 0, 400 + s, 512)[key]
+        adj.append(float(np.sum(g.cpu().numpy() * delta)))
+    fds, adj = np.array(fds), np.array(adj)
+    se = np.sqrt(fds.var(ddof=1) / len(fds) + adj.var(ddof=1) / len(adj))
+    assert abs(fds.mean() - adj.mean()) < 4 * se + 0.02 * abs(fds.mean()), (fds.mean(), adj.mean(), se)
+
+
+def test_albedo_grid_adjoint_matches_finite_differences(cuda):
+    # dL/d(albedo grid (n, n, n, 3)) by path replay vs two-state central differences
+    announce("albedo adjoint vs FD")
+    from recursive_control_variates_amd.transform import T
+    rng = np.random.default_rng(12)
+    n = 4
+    dens = (0.4 + rng.uniform(0, 0.6, (n, n, n, 1))).astype(np.float32)
+    alb = rng.uniform(0.4, 0.9, (n, n, n, 3)).astype(np.float32)
+    alb_dict = {"type": "gridvolume", "data": alb, "to_world": T.scale(1.0)}
+    sc = rcv.load_dict(volume_box_scene(dens, albedo=alb_dict, scale=5.0, w=8, h=8, spp=16))
+    key = "box.interior_medium.albedo.data"
+    gimg = rng.uniform(0.5, 1.5, (8, 8, 3))
+    delta = rng.uniform(-1, 1, alb.shape).astype(np.float32)
+    eps = 2e-2
+    ts = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 8})
+    plain = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    p = rcv.traverse(sc)
+    fds, adj = [], []
+    for s in range(6):
+        p[key] = torch.from_numpy(alb + eps * delta).cuda()
+        p.update()
+        p[key] = torch.from_numpy(alb - eps * delta).cuda()
+        p.update()
+        H, F = ts.render_twostates(sc, 0, 800 + s, 1024)
+        fds.append(float(np.sum(gimg * (H - F).cpu().numpy()) / (2 * eps)))
+        x = torch.from_numpy(alb).cuda().requires_grad_(True)
+        p[key] = x
+        p.update()
+        p.update()
+        g = plain.adjoint(sc, p, torch.from_numpy(gimg.astype(np.float32)).cuda(), 0, 900 + s, 1024)[key]
         adj.append(float(np.sum(g.cpu().numpy() * delta)))
     fds, adj = np.array(fds), np.array(adj)
     se = np.sqrt(fds.var(ddof=1) / len(fds) + adj.var(ddof=1) / len(adj))

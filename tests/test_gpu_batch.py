@@ -18,11 +18,11 @@ VKEY = "box.interior_medium.sigma_t.data"
 TKEY = "back.bsdf.reflectance.data"
 
 
-def vol_spec(seed, n=6, w=12, h=10, spp=8, method="cv_ps", max_depth=8, twostate=True):
+def vol_spec(seed, n=6, w=12, h=10, spp=8, method="cv_ps", max_depth=8, twostate=True, majorant_factor=0):
     rng = np.random.default_rng(seed)
     target = rng.uniform(0.2, 1.0, (n, n, n, 1)).astype(np.float32)
     init = np.full_like(target, 0.5)
-    d = volume_box_scene(init, scale=5.0, w=w, h=h, spp=spp, twostate=twostate)
+    d = volume_box_scene(init, scale=5.0, w=w, h=h, spp=spp, twostate=twostate, majorant_factor=majorant_factor)
     sc = rcv.load_dict(d)
     p = rcv.traverse(sc)
     p[VKEY] = torch.from_numpy(target).cuda()
@@ -80,6 +80,22 @@ def test_batch_of_one_follows_driver(cuda):
     np.testing.assert_allclose(b.param(0).cpu().numpy(), ref_param.cpu().numpy(), rtol=1e-3, atol=1e-4)
     it, _ = b.job_state(0)
     assert it == 3
+
+
+def test_supergrid_job_follows_driver(cuda):
+    # the batch rebuilds the optimised grid's supergrids after every rotation,
+    # as the single-job path does when the parameter is rebound
+    announce("batch supergrid job vs driver")
+    mk = lambda: vol_spec(11, n=8, w=12, h=10, spp=8, majorant_factor=2)
+    ref_param, ref_imgs = driver_run(mk(), 3)
+    b = BatchOptimizer([mk()], use_graph=True)
+    b.step(1)
+    b.synchronize()
+    np.testing.assert_allclose(b.image(0).cpu().numpy(), ref_imgs[0].cpu().numpy(), rtol=1e-5, atol=1e-6)
+    b.step(2)
+    b.synchronize()
+    _params_close(b.param(0), ref_param, 0.02, 3)
+    assert b.job_state(0)[0] == 3
 
 
 def _grads_close(a, b):

@@ -1,0 +1,81 @@
+"""Continuous-batching service: a job's trajectory is the same alone and when
+other jobs join mid-run (the batch is rebuilt with state carried over on the
+device); admission respects the HBM budget; a short Poisson-arrival run
+reports latency percentiles and throughput."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce
+from test_gpu_batch import surf_spec, vol_spec
+from recursive_control_variates_amd.serve import Server, run_simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def _drain(srv, t=0.0):
+    while any(j.pending for j in srv.jobs):
+        srv.engine_step(t)
+
+
+def test_job_same_alone_and_with_joiners(cuda):
+    announce("serve alone vs joiners")
+    meta_v = {"kind": "volume", "side": 12, "grid": 6}
+    meta_s = {"kind": "surface", "side": 16, "grid": 64}
+    results = []
+    for with_others in (False, True):
+        srv = Server(budget_bytes=1e9)
+        a = srv.submit_job(lambda: (vol_spec(21, w=12, h=10, spp=8), {}), 6, meta_v)
+        srv.engine_step(0.0)                       # admits A
+        imgs = []
+        for it in range(6):
+            if with_others and it == 2:
+                srv.submit_job(lambda: (surf_spec(22, w=16, h=12, spp=8), {}), 4, meta_s)
+                srv.submit_job(lambda: (vol_spec(23, n=5, w=8, h=8, spp=4), {}), 4, meta_v)
+                srv.engine_step(0.0)               # admits, rebuilds the batch (no pending work)
+                for j in srv.jobs[1:]:
+                    for _ in range(4):
+                        srv.submit_step(j.jid, 0.0)
+            assert srv.submit_step(a, 0.0)
+            srv.engine_step(0.0)
+            imgs.append(srv.jobs[0].buffers.img.detach().clone())
+        _drain(srv)
+        assert srv.jobs[0].done == 6
+        results.append((imgs, srv.batch.param(0).detach().clone() if srv.batch else None))
+        srv.close()
+    (ia, pa), (ib, pb) = results
+    # identical first iteration (no gradient has been applied yet); afterwards
+    # only the float-atomic order of the gradient sums differs
+    assert torch.equal(ia[0], ib[0])
+    for x, y in zip(ia[1:], ib[1:]):
+        np.testing.assert_allclose(x.cpu().numpy(), y.cpu().numpy(), rtol=0.05, atol=0.02)
+    d = np.abs(pa.cpu().numpy() - pb.cpu().numpy())
+    assert d.max() <= 2 * 0.02 * 6 and np.median(d) < 1e-3
+
+
+def test_admission_respects_budget(cuda):
+    announce("serve admission")
+    meta = {"kind": "volume", "side": 12, "grid": 6}
+    from recursive_control_variates_amd.serve import footprint_estimate
+    fp = footprint_estimate({"kind": "volume", "side": 12, "grid": 6})
+    srv = Server(budget_bytes=2.5 * fp)
+    ids = [srv.submit_job(lambda s=s: (vol_spec(30 + s, spp=4), {}), 2, meta) for s in range(4)]
+    srv.engine_step(0.0)
+    assert len(srv.jobs) == 2 and len(srv.waiting) == 2
+    for j in list(srv.jobs):
+        for _ in range(2):
+            srv.submit_step(j.jid, 0.0)
+    _drain(srv)
+    srv.engine_step(0.0)                           # retires both, admits the waiting two
+    assert len(srv.finished) == 2 and len(srv.jobs) == 2 and not srv.waiting
+    assert srv.used <= srv.budget
+    srv.close()
+    assert ids == [0, 1, 2, 3]
+
+
+def test_poisson_simulation_reports(cuda):
+    announce("serve simulation")
+    out = run_simulation(duration=3.0, rate=200.0, target_jobs=3, iterations=(5, 10), seed=3)
+    assert out["served_steps"] > 0 and out["p50_step_ms"] is not None
+    assert out["p99_step_ms"] >= out["p50_step_ms"] > 0
+    assert out["msamples_per_s"] > 0 and out["jobs_admitted"] >= 3

@@ -135,3 +135,74 @@ def test_scene_checks_and_errors(cuda):
     t.add_(0.5)                       # in-place write to a bound state is rejected
     with pytest.raises(RuntimeError, match="modified in place"):
         ts.render_twostates(sc, 0, 1, 4)
+
+
+def test_supergrid_matches_oracle_and_follows_updates(cuda):
+    # majorant supergrid built on the device equals the oracle's cell maxima
+    # (a max of the same floats: exact), for a ragged grid, and is rebuilt
+    # for the NEW state when the parameter changes
+    announce("supergrid build vs oracle")
+    rng = np.random.default_rng(40)
+    ga = rng.uniform(0, 1, (5, 7, 10, 1)).astype(np.float32)
+    gb = rng.uniform(0, 2, (5, 7, 10, 1)).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(ga, scale=4.0, majorant_factor=3))
+    assert np.array_equal(sc.majorant_grid(0, 1), R.majorant_grid(sc.desc, [ga]))
+    p = rcv.traverse(sc)
+    p["box.interior_medium.sigma_t.data"] = torch.from_numpy(gb).cuda()
+    p.update()                       # old = ga, new = gb
+    assert np.array_equal(sc.majorant_grid(0, 0), R.majorant_grid(sc.desc, [ga]))
+    assert np.array_equal(sc.majorant_grid(0, 1), R.majorant_grid(sc.desc, [gb]))
+    plain = rcv.load_dict(volume_box_scene(ga, scale=4.0))
+    assert plain.majorant_grid(0, 1) is None
+
+
+@pytest.mark.parametrize("factor", [2, 3])
+def test_supergrid_two_state_matches_oracle(cuda, factor):
+    # DDA free flight through the combined (max of both states) supergrid,
+    # sample by sample against the float32 oracle
+    announce(f"supergrid two-state vs oracle fp32 (factor {factor})")
+    rng = np.random.default_rng(41)
+    ga = rng.uniform(0, 1, (8, 8, 8, 1)).astype(np.float32)
+    ga[:, :, :3] *= 0.1
+    gb = (ga * rng.uniform(0.5, 1.6, ga.shape)).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(ga, scale=6.0, w=16, h=12, spp=8, majorant_factor=factor))
+    p = rcv.traverse(sc)
+    p["box.interior_medium.sigma_t.data"] = torch.from_numpy(gb).cuda()
+    p.update()
+    integ = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 3})
+    H, F = integ.render_twostates(sc, 0, 321, 8)
+    Ho, Fo = R.render(sc.desc, VOL, 8, 3, 321, 8, params_old=[ga], params_new=[gb], precision=32)
+    for g, o in ((H, Ho), (F, Fo)):
+        g = g.cpu().numpy()
+        assert _close_fraction(g, o) > 0.97
+        assert abs(g.mean() - o.mean()) < 2e-3 * abs(o.mean()) + 1e-5
+
+
+def test_supergrid_plain_equals_new_state_and_is_unbiased(cuda):
+    # plain render (NEW supergrid only) == two-state F with equal states, and
+    # the supergrid tracker agrees with the global-majorant tracker per pixel
+    announce("supergrid plain vs global majorant")
+    rng = np.random.default_rng(42)
+    g = rng.uniform(0, 1, (8, 8, 8, 1)).astype(np.float32)
+    g[2:5, 3:6, 1:4] *= 4.0
+    sg = rcv.load_dict(volume_box_scene(g, scale=5.0, w=10, h=8, spp=64, majorant_factor=2))
+    gl = rcv.load_dict(volume_box_scene(g, scale=5.0, w=10, h=8, spp=64))
+    ts = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 8})
+    plain = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    H, F = ts.render_twostates(sg, 0, 17, 64)
+    assert torch.equal(H, F)
+    assert torch.equal(plain.render(sg, 0, 17, 64), F)
+    a = np.stack([plain.render(sg, 0, 100 + s, 256).cpu().numpy() for s in range(8)])
+    b = np.stack([plain.render(gl, 0, 200 + s, 256).cpu().numpy() for s in range(8)])
+    se = np.sqrt(a.var(0, ddof=1) / 8 + b.var(0, ddof=1) / 8) + 1e-7
+    z = (a.mean(0) - b.mean(0)) / se
+    assert np.mean(np.abs(z[..., 0]) > 3.5) < 0.03
+    assert abs(z[..., 0].mean()) < 4.0 / np.sqrt(z[..., 0].size)
+    sg.set_counting(True)
+    gl.set_counting(True)
+    plain.render(sg, 0, 5, 64)
+    c_sg = sg.read_counters()
+    plain.render(gl, 0, 5, 64)
+    c_gl = gl.read_counters()
+    # the supergrid removes null collisions in the thin cells
+    assert c_sg["null_events"] < c_gl["null_events"]
