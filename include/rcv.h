@@ -98,6 +98,7 @@ const char *rcv_last_error(void);
  * (incl. 'twostatemedium', plugins/twostatemedium.py:5-200), emitters and the
  * (batch) sensor. Build with rcv_scene_add_*, then rcv_scene_commit. */
 int rcv_scene_create(rcv_scene **out);
+/* RCV_E_STATE (and nothing freed) while a batch holds the scene. */
 int rcv_scene_destroy(rcv_scene *scene);
 
 /* Declares a differentiable parameter (texture or grid). */
@@ -110,6 +111,20 @@ int rcv_scene_add_param(rcv_scene *scene, int32_t kind, int32_t nx, int32_t ny,
  * (majorant, plugins/twostatemedium.py:73-75) is reduced on `stream`. */
 int rcv_scene_bind_param(rcv_scene *scene, int32_t param, int32_t state,
                          const float *device_data, void *stream);
+/* Changes the resolution of grid/texture parameter `param` after commit (the
+ * resolution change of the upsampling step, utils.py:177-192, where the
+ * Mitsuba volume or bitmap behind the key is replaced by a larger one). Both
+ * states become unbound and must be bound again at the new size; majorant
+ * supergrids of media over the grid are re-planned. RCV_E_STATE while a
+ * batch holds the scene. */
+int rcv_scene_resize_param(rcv_scene *scene, int32_t param, int32_t nx, int32_t ny,
+                           int32_t nz, void *stream);
+/* Medium.set_majorant_resolution_factor (plugins/twostatemedium.py:174-178,
+ * called by adjust_majorant_res_factor, utils.py:27-43): 0 = one global
+ * majorant, k > 0 = supergrid of ceil(n / k) cells per axis, rebuilt for the
+ * bound states. RCV_E_STATE while a batch holds the scene. */
+int rcv_scene_set_majorant_factor(rcv_scene *scene, int32_t medium, int32_t factor,
+                                  void *stream);
 
 int rcv_scene_add_bsdf(rcv_scene *scene, int32_t type, const float rgb[3],
                        int32_t texture_param, int32_t *out_id);
@@ -213,7 +228,9 @@ int rcv_cv_step(const rcv_cv_state *state, const float *H, const float *F,
                 int64_t n_elem, int32_t warmup, float beta1, float beta2,
                 int32_t pre_update, void *stream);
 /* WelfordVarianceEstimator.update (plugins/welford.py:27-33): mean/var arrays
- * of n_elem, counter[0] = n. */
+ * of n_elem. `counter` points to two device int32: counter[0] = n (advanced
+ * on the device), counter[1] = scratch of the last-block election; both
+ * start at zero. */
 int rcv_welford_update(float *mean, float *var, int32_t *counter,
                        const float *x, int64_t n_elem, float beta1,
                        float beta2, void *stream);

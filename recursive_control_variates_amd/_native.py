@@ -92,6 +92,8 @@ SIGNATURES = {
     "rcv_scene_set_counting": [_P, _I],
     "rcv_scene_read_counters": [_P, C.POINTER(_I64), _I, _P],
     "rcv_scene_majorant_grid": [_P, _I, _I, _P, _I64, C.POINTER(_I), _P],
+    "rcv_scene_resize_param": [_P, _I, _I, _I, _I, _P],
+    "rcv_scene_set_majorant_factor": [_P, _I, _I, _P],
     "rcv_render": [_P, C.POINTER(IntegratorDesc), _I, _U32, _I, _P, _P],
     "rcv_render_twostates": [_P, C.POINTER(IntegratorDesc), _I, _U32, _I, _P, _P, _P],
     "rcv_render_backward": [_P, C.POINTER(IntegratorDesc), _I, _U32, _I, _P, _I, C.POINTER(_I),

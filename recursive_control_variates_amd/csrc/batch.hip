@@ -418,6 +418,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   ~Batch() {
+    for (const Scene *s : scenes) --const_cast<Scene *>(s)->batch_refs;
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (dev) (void)hipFree(dev);
@@ -532,6 +533,7 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
   RCV_REQUIRE(n_sg <= 1, "rcv_batch_add_job: the optimised grid drives %d media with supergrids (at most 1)", n_sg);
   b->host_jobs.push_back(J);
   b->scenes.push_back(s);
+  ++const_cast<Scene *>(s)->batch_refs;
   if (out_job) *out_job = int(b->host_jobs.size()) - 1;
   return RCV_OK;
 }

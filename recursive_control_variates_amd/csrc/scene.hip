@@ -284,20 +284,7 @@ int Scene::upload(hipStream_t stream) {
     bs_radius = 0.5f * std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   }
 
-  // ---- medium supergrid offsets
-  int sg_total = 0;
-  for (auto &m : media) {
-    m.supergrid_offset = -1;
-    if (m.majorant_factor > 0) {
-      const Param &p = params[m.sigma_param];
-      for (int k = 0; k < 3; ++k) {
-        const int n = k == 0 ? p.nx : (k == 1 ? p.ny : p.nz);
-        m.sg_res[k] = std::max(1, (n + m.majorant_factor - 1) / m.majorant_factor);
-      }
-      m.supergrid_offset = sg_total;
-      sg_total += 2 * m.sg_res[0] * m.sg_res[1] * m.sg_res[2];
-    }
-  }
+  plan_supergrids();
 
   // ---- pack one device block
   std::vector<char> blob;
@@ -322,8 +309,6 @@ int Scene::upload(hipStream_t stream) {
   const size_t o_sens = put(sensors.data(), sensors.size() * sizeof(SensorInfo));
   std::vector<uint32_t> maj_init(2 * kMaxParams, float_key(-kInf));
   const size_t o_maj = put(maj_init.data(), maj_init.size() * 4);
-  std::vector<float> sg_init(std::max(1, sg_total), 0.f);
-  const size_t o_sg = put(sg_init.data(), sg_init.size() * 4);
   std::vector<unsigned long long> cnt(kNumCounters, 0ull);
   const size_t o_cnt = put(cnt.data(), cnt.size() * 8);
 
@@ -345,12 +330,11 @@ int Scene::upload(hipStream_t stream) {
   v.emitters = reinterpret_cast<const EmitterInfo *>(base + o_em);
   v.em_cdf = reinterpret_cast<const float *>(base + o_cdf);
   v.em_tris = reinterpret_cast<const int32_t *>(base + o_emt);
-  v.media = reinterpret_cast<const MediumInfo *>(base + o_med);
+  media_dev = reinterpret_cast<MediumInfo *>(base + o_med);
+  v.media = media_dev;
   v.sensors = reinterpret_cast<const SensorInfo *>(base + o_sens);
   majorant_keys = reinterpret_cast<uint32_t *>(base + o_maj);
   v.majorants = reinterpret_cast<const float *>(majorant_keys);
-  supergrids = reinterpret_cast<float *>(base + o_sg);
-  v.supergrids = supergrids;
   counters = reinterpret_cast<unsigned long long *>(base + o_cnt);
   v.counters = nullptr;
   v.n_tris = n_tris;
@@ -373,6 +357,7 @@ int Scene::upload(hipStream_t stream) {
     }
   }
   n_nodes = int(b.nodes.size());
+  if (int rc = alloc_supergrids()) return rc;
   committed = true;
   // Majorants of states bound before commit.
   for (int i = 0; i < int(params.size()); ++i)
@@ -404,8 +389,67 @@ int Scene::reduce_param(int param, int state, hipStream_t stream) {
   return RCV_OK;
 }
 
+// Supergrid resolution (ceil(n / factor) cells per axis) and storage offset
+// of every medium with a majorant factor; returns the floats needed.
+int64_t Scene::plan_supergrids() {
+  int64_t total = 0;
+  for (auto &m : media) {
+    m.supergrid_offset = -1;
+    m.sg_res[0] = m.sg_res[1] = m.sg_res[2] = 0;
+    if (m.majorant_factor > 0) {
+      const Param &p = params[m.sigma_param];
+      for (int k = 0; k < 3; ++k) {
+        const int n = k == 0 ? p.nx : (k == 1 ? p.ny : p.nz);
+        m.sg_res[k] = std::max(1, (n + m.majorant_factor - 1) / m.majorant_factor);
+      }
+      m.supergrid_offset = int32_t(total);
+      total += 2 * int64_t(m.sg_res[0]) * m.sg_res[1] * m.sg_res[2];
+    }
+  }
+  return total;
+}
+
+// (Re)allocates the supergrid storage for the current plan (grown only).
+int Scene::alloc_supergrids() {
+  int64_t total = 0;
+  for (auto &m : media)
+    if (m.supergrid_offset >= 0) total += 2 * int64_t(m.sg_res[0]) * m.sg_res[1] * m.sg_res[2];
+  RCV_REQUIRE(total < (int64_t(1) << 31), "majorant supergrids: %lld cells exceed 2^31", (long long)total);
+  if (total > sg_capacity) {
+    if (sg_block) RCV_TRY_HIP(hipFree(sg_block), "majorant supergrids: hipFree");
+    sg_block = nullptr;
+    sg_capacity = 0;
+    RCV_TRY_HIP(hipMalloc(&sg_block, size_t(total) * sizeof(float)), "majorant supergrids: hipMalloc");
+    sg_capacity = total;
+  }
+  supergrids = static_cast<float *>(sg_block);
+  view.supergrids = supergrids;
+  return RCV_OK;
+}
+
+// After a change of a grid's resolution or of a majorant factor: new
+// supergrid layout, media table re-uploaded, supergrids of bound states
+// rebuilt.
+int Scene::relayout_supergrids(hipStream_t stream) {
+  plan_supergrids();
+  if (int rc = alloc_supergrids()) return rc;
+  RCV_TRY_HIP(hipMemcpyAsync(media_dev, media.data(), media.size() * sizeof(MediumInfo), hipMemcpyHostToDevice,
+                             stream),
+              "majorant supergrids: media upload");
+  for (int mi = 0; mi < int(media.size()); ++mi) {
+    if (media[mi].supergrid_offset < 0) continue;
+    for (int s = 0; s < 2; ++s)
+      if (params[media[mi].sigma_param].data[s])
+        if (int rc = build_supergrid(mi, s, stream)) return rc;
+  }
+  // the host copy of the media table must outlive the upload
+  RCV_TRY_HIP(hipStreamSynchronize(stream), "majorant supergrids: synchronize");
+  return RCV_OK;
+}
+
 Scene::~Scene() {
   if (dev_block) (void)hipFree(dev_block);
+  if (sg_block) (void)hipFree(sg_block);
 }
 
 }  // namespace rcv
@@ -423,6 +467,9 @@ extern "C" int rcv_scene_create(rcv_scene **out) {
 }
 
 extern "C" int rcv_scene_destroy(rcv_scene *scene) {
+  if (scene && scene->batch_refs > 0)
+    return fail(RCV_E_STATE, "rcv_scene_destroy: the scene belongs to %d batch(es); destroy them first",
+                scene->batch_refs);
   delete scene;
   return RCV_OK;
 }
@@ -462,6 +509,37 @@ extern "C" int rcv_scene_bind_param(rcv_scene *s, int32_t param, int32_t state, 
   if (!s->committed) return RCV_OK;
   s->view.params[param].data[state] = data;
   return s->reduce_param(param, state, as_stream(stream));
+}
+
+extern "C" int rcv_scene_resize_param(rcv_scene *s, int32_t param, int32_t nx, int32_t ny, int32_t nz,
+                                      void *stream) {
+  RCV_REQUIRE(s && s->committed, "rcv_scene_resize_param: scene not committed");
+  RCV_REQUIRE(param >= 0 && param < int(s->params.size()), "rcv_scene_resize_param: bad param id %d", param);
+  if (s->batch_refs > 0) return fail(RCV_E_STATE, "rcv_scene_resize_param: the scene belongs to a batch");
+  Param &p = s->params[param];
+  RCV_REQUIRE(nx > 0 && ny > 0 && nz > 0, "rcv_scene_resize_param: sizes must be positive");
+  RCV_REQUIRE(p.kind != RCV_PARAM_TEXTURE || nz == 1, "rcv_scene_resize_param: a texture has nz = 1");
+  RCV_REQUIRE(int64_t(nx) * ny * nz * p.ch < (int64_t(1) << 31), "rcv_scene_resize_param: too many elements");
+  p.nx = nx;
+  p.ny = ny;
+  p.nz = nz;
+  p.data[0] = p.data[1] = nullptr;
+  ParamBind &v = s->view.params[param];
+  v.nx = nx;
+  v.ny = ny;
+  v.nz = nz;
+  v.data[0] = v.data[1] = nullptr;
+  return s->relayout_supergrids(as_stream(stream));
+}
+
+extern "C" int rcv_scene_set_majorant_factor(rcv_scene *s, int32_t medium, int32_t factor, void *stream) {
+  RCV_REQUIRE(s && s->committed, "rcv_scene_set_majorant_factor: scene not committed");
+  RCV_REQUIRE(medium >= 0 && medium < int(s->media.size()), "rcv_scene_set_majorant_factor: bad medium %d",
+              medium);
+  RCV_REQUIRE(factor >= 0, "rcv_scene_set_majorant_factor: factor must be >= 0");
+  if (s->batch_refs > 0) return fail(RCV_E_STATE, "rcv_scene_set_majorant_factor: the scene belongs to a batch");
+  s->media[medium].majorant_factor = factor;
+  return s->relayout_supergrids(as_stream(stream));
 }
 
 extern "C" int rcv_scene_add_bsdf(rcv_scene *s, int32_t type, const float rgb[3], int32_t texture_param,

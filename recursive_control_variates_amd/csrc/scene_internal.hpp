@@ -33,7 +33,11 @@ struct Scene {
   bool committed = false;
   void *dev_block = nullptr;
   uint32_t *majorant_keys = nullptr;  // [param][state], order-preserving float keys
-  float *supergrids = nullptr;
+  float *supergrids = nullptr;      // sg_block: [medium][state][cell]
+  void *sg_block = nullptr;
+  int64_t sg_capacity = 0;          // floats
+  MediumInfo *media_dev = nullptr;  // device copy of `media`
+  int batch_refs = 0;               // batches holding this scene (no relayout then)
   unsigned long long *counters = nullptr;
   int n_nodes = 0;
   float bs_center[3] = {0, 0, 0};
@@ -48,6 +52,9 @@ struct Scene {
   int upload(hipStream_t stream);
   int reduce_param(int param, int state, hipStream_t stream);
   int build_supergrid(int medium, int state, hipStream_t stream);
+  int64_t plan_supergrids();
+  int alloc_supergrids();
+  int relayout_supergrids(hipStream_t stream);
 };
 
 // Decodes an order-preserving key written by the majorant reduction.

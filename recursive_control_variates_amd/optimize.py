@@ -5,7 +5,9 @@ Per iteration i: parameters_changed (scene update from the optimizer), render
 with the meta integrator (seed i, spp, spp_grad), loss against the reference
 image, checkpoint every ``recomp_freq`` steps (plain render at ``spp_inf``,
 seed i+1), backward through the path-replay adjoint, Adam step with fused
-clamping, learning-rate halving at the ``schedule`` fractions.
+clamping, resolution doubling at the ``upsample`` fractions (with the CV
+reset and the supergrid factor update), learning-rate halving at the
+``schedule`` fractions.
 
 Extras for measurement: if ``target`` ({key: tensor}) is given, the
 parameter MSE against it is recorded every step; ``benchmark`` records the
@@ -23,7 +25,7 @@ import torch
 from .cv_integrator import MetaIntegrator
 from .optim import Adam, l1_loss, l2_loss
 from .rendering import render
-from .scene import traverse
+from .scene import resample_nearest, traverse
 
 
 def _loss_func(name):
@@ -59,13 +61,56 @@ def render_reference(cfg, scene, integrator, params=None, target=None):
     return img
 
 
+def _single_medium(scene) -> int:
+    """Index of the scene's only medium (utils.py get_single_medium)."""
+    n = len(scene.desc.media)
+    if n != 1:
+        raise ValueError(f"expected exactly one medium in the scene, found {n}")
+    return 0
+
+
+def adjust_majorant_res_factor(scene, density_res):
+    """utils.py:27-43: the largest supergrid factor <= 8 that leaves at least
+    4 cells along the grid's shortest side; 0 (one global majorant) when
+    none does."""
+    res_factor = 8
+    min_side = min(int(x) for x in density_res[:3])
+    while res_factor > 1 and (min_side // res_factor) < 4:
+        res_factor -= 1
+    if res_factor <= 1:
+        res_factor = 0
+    medium = _single_medium(scene)
+    if scene.majorant_resolution_factor(medium) != res_factor:
+        scene.set_majorant_factor(medium, res_factor)
+
+
+def upsample(scene, cfg, opt, integrator):
+    """utils.py:177-192: every optimised tensor doubles its resolution by
+    voxel repetition (Adam's state for it restarts, as for any shape change),
+    the supergrid factor follows a density's new resolution, and the CV state
+    is reset."""
+    use_sg = bool(cfg.get("use_majorant_supergrid", False))
+    for key in cfg["variables"].keys():
+        x = opt[key]
+        if not isinstance(x, torch.Tensor) or x.dim() not in (3, 4):
+            raise ValueError(f"Upsampling expects a 3 or 4D tensor, got {getattr(x, 'shape', type(x))}")
+        new_res = (*[2 * s for s in x.shape[:-1]], x.shape[-1])
+        opt[key] = resample_nearest(x.detach(), new_res)
+        if ".sigma_t." in key and use_sg:
+            adjust_majorant_res_factor(scene, new_res)
+    integrator.reset()
+
+
 def initialize_parameters(cfg, opt, scene_params):
-    """utils.py:62-87: run twice for cv_ps so that old = new = init."""
+    """utils.py:62-87: run twice for cv_ps so that old = new = init; a grid
+    initial density sets the supergrid factor when use_majorant_supergrid."""
     for _ in range(2 if cfg.get("method", "baseline") == "cv_ps" else 1):
         for key, p in cfg["variables"].items():
             init = p["init"]
             if not isinstance(init, torch.Tensor):
                 init = torch.as_tensor(np.asarray(init, np.float32))
+            if "sigma_t" in key and init.dim() == 4 and cfg.get("use_majorant_supergrid", False):
+                adjust_majorant_res_factor(scene_params.scene, tuple(init.shape))
             opt[key] = init.to(scene_params.scene.device)
             if "lr_factor" in p:
                 opt.set_learning_rate({key: cfg.get("lr", 1e-2) * p["lr_factor"]})
@@ -89,8 +134,6 @@ def run_opt(cfg: Dict) -> Dict:
     mse_threshold = cfg.get("mse_threshold")
     if "integrator" not in cfg:
         raise ValueError("No integrator specified")
-    if cfg.get("upsample"):
-        raise NotImplementedError("resolution upsampling during optimisation is not part of this build")
     base = cfg["integrator"]
     integrator = MetaIntegrator({
         "method": method, "integrator": base,
@@ -103,7 +146,16 @@ def run_opt(cfg: Dict) -> Dict:
     if img_ref is None:
         img_ref = render_reference(cfg, scene, base, params, target)
 
-    schedule_steps = [int(n_steps * w) for w in cfg.get("schedule", [])]
+    upsample_steps = []
+    for w in cfg.get("upsample", []):
+        if not 0 < w < 1:
+            raise ValueError("upsample fractions must lie in (0, 1)")
+        upsample_steps.append(int(n_steps * w))
+    schedule_steps = []
+    for w in cfg.get("schedule", []):
+        if not 0 < w < 1:
+            raise ValueError("schedule fractions must lie in (0, 1)")
+        schedule_steps.append(int(n_steps * w))
     result = {"loss": np.zeros(1 + n_steps // max(1, recomp_freq), np.float32),
               "var": np.zeros(1 + n_steps // max(1, recomp_freq), np.float32),
               "step_ms": np.zeros(n_steps, np.float64),
@@ -127,6 +179,8 @@ def run_opt(cfg: Dict) -> Dict:
                 result["var"][i // recomp_freq] = float(((img_inf - img.detach()) ** 2).mean())
         loss.backward()
         opt.step()
+        if i in upsample_steps:
+            upsample(scene, cfg, opt, integrator)
         if i in schedule_steps:
             lr *= 0.5
             opt.set_learning_rate(lr)
@@ -140,7 +194,10 @@ def run_opt(cfg: Dict) -> Dict:
             cnt = 0
             with torch.no_grad():
                 for k, v in target.items():
-                    d = opt[k].detach() - v.to(opt[k].device)
+                    est = opt[k].detach()
+                    if est.shape != v.shape:       # coarse phase of an upsampled run
+                        est = resample_nearest(est, tuple(v.shape))
+                    d = est - v.to(est.device)
                     mse += float((d * d).sum())
                     cnt += d.numel()
             result["param_mse"][i] = mse / cnt

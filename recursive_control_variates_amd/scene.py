@@ -145,20 +145,52 @@ class Scene:
             value = torch.as_tensor(np.asarray(value, np.float32))
         value = value.detach().to(self.device, torch.float32)
         if tuple(value.shape) != tuple(sp.shape):
-            raise ValueError(f"{key}: shape {tuple(value.shape)} does not match {sp.shape} "
-                             "(resolution changes need a new scene)")
+            if value.dim() != len(sp.shape) or value.shape[-1] != sp.shape[-1]:
+                raise ValueError(f"{key}: shape {tuple(value.shape)} is not a resolution change of {sp.shape}")
         sp.incoming = value.contiguous()
 
     def parameters_changed(self, keys):
         """old <- new <- incoming for two-state parameters; plain parameters
-        take the incoming value for both states."""
+        take the incoming value for both states.
+
+        An incoming value of another resolution (upsampling, utils.py:177-192)
+        resizes the parameter. Both states of one parameter share one
+        resolution here, so the state that becomes OLD is the previous NEW
+        state resampled to the new resolution by voxel repetition (upstream
+        keeps the coarse volume as OLD for one iteration; that iteration's
+        OLD image is not used, because upsampling also resets the CV state)."""
         for k in keys:
             sp = self.params[k]
+            if tuple(sp.incoming.shape) != tuple(sp.shape):
+                self._resize(sp, tuple(sp.incoming.shape))
             if sp.twostate:
                 sp.states = [sp.states[N.STATE_NEW], sp.incoming]
             else:
                 sp.states = [sp.incoming, sp.incoming]
             self._bind(sp)
+
+    def _resize(self, sp: SceneParam, shape: tuple):
+        if sp.kind == N.PARAM_TEXTURE:
+            ny, nx, _ = shape
+            nz = 1
+        else:
+            nz, ny, nx, _ = shape
+        N.call("rcv_scene_resize_param", self._handle, sp.index, nx, ny, nz,
+               C.c_void_p(N.stream_handle(self.device)))
+        sp.shape = shape
+        self.desc.params[sp.index].shape = shape
+        sp.states = [resample_nearest(t, shape) if t is not None else None for t in sp.states]
+
+    def set_majorant_factor(self, medium: int, factor: int):
+        """Medium.set_majorant_resolution_factor (plugins/twostatemedium.py:
+        174-178): 0 = one global majorant, k > 0 = supergrid of ceil(n / k)
+        cells per axis."""
+        N.call("rcv_scene_set_majorant_factor", self._handle, medium, int(factor),
+               C.c_void_p(N.stream_handle(self.device)))
+        self.desc.media[medium].majorant_factor = int(factor)
+
+    def majorant_resolution_factor(self, medium: int = 0) -> int:
+        return int(self.desc.media[medium].majorant_factor)
 
     def check_params(self):
         for sp in self._param_list:
@@ -215,6 +247,15 @@ class Scene:
 
     def __del__(self):
         self._destroy()
+
+
+def resample_nearest(t: torch.Tensor, shape: tuple) -> torch.Tensor:
+    """Nearest-neighbour resampling of a (ny, nx, c) texture or (nz, ny, nx, c)
+    grid to ``shape`` (voxel repetition for integer factors, the tensor case
+    of dr.upsample used at utils.py:184)."""
+    x = t.movedim(-1, 0).unsqueeze(0)
+    y = torch.nn.functional.interpolate(x, size=tuple(shape[:-1]), mode="nearest")
+    return y.squeeze(0).movedim(0, -1).contiguous()
 
 
 class SceneParameters:

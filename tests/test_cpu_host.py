@@ -154,3 +154,29 @@ def test_adam_host_state():
     opt["a"] = torch.ones(3)        # same shape: state kept
     opt["a"] = torch.ones(4)        # new shape: state reset
     assert opt.state["a"][0].shape == (4,)
+
+
+def test_adjust_majorant_res_factor_rule():
+    # utils.py:27-43: largest factor <= 8 with >= 4 cells on the shortest side
+    from recursive_control_variates_amd.optimize import adjust_majorant_res_factor
+    from recursive_control_variates_amd.scene_desc import MediumDesc
+
+    class _Scene:
+        def __init__(self):
+            self.desc = type("D", (), {})()
+            self.desc.media = [MediumDesc(0, -1, (0.8,) * 3, 1.0, (0,) * 3, (1,) * 3, True, 0, False)]
+            self.calls = []
+
+        def majorant_resolution_factor(self, m):
+            return self.desc.media[m].majorant_factor
+
+        def set_majorant_factor(self, m, k):
+            self.calls.append(k)
+            self.desc.media[m].majorant_factor = k
+
+    expect = {4: 0, 7: 0, 8: 2, 11: 2, 12: 3, 16: 4, 20: 5, 31: 7, 32: 8, 64: 8, 256: 8}
+    for side, k in expect.items():
+        s = _Scene()
+        adjust_majorant_res_factor(s, (side, side + 5, side + 9, 1))
+        assert s.majorant_resolution_factor(0) == k, (side, k)
+        assert s.calls == ([] if k == 0 else [k])

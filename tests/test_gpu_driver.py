@@ -1,0 +1,81 @@
+"""The optimisation driver (run_opt, optimize.py:10-154 with utils.py:27-192):
+parameter resolution changes through the scene, upsampling with the CV
+reset and the supergrid factor, the learning-rate schedule and the
+checkpoint arrays."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce
+from helpers import volume_box_scene
+import recursive_control_variates_amd as rcv
+from recursive_control_variates_amd.optimize import run_opt
+from recursive_control_variates_amd.scene import resample_nearest
+
+pytestmark = pytest.mark.gpu
+KEY = "box.interior_medium.sigma_t.data"
+
+
+@pytest.mark.parametrize("factor", [0, 2])
+def test_resized_parameter_renders_like_a_fresh_scene(cuda, factor):
+    # a scene whose grid went 6^3 -> 12^3 through parameter updates renders
+    # bit for bit like a scene loaded with the 12^3 grid
+    announce(f"resize vs fresh scene (majorant factor {factor})")
+    rng = np.random.default_rng(50)
+    coarse = rng.uniform(0, 1, (6, 6, 6, 1)).astype(np.float32)
+    fine = rng.uniform(0, 1, (12, 12, 12, 1)).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(coarse, scale=5.0, w=12, h=10, spp=8, majorant_factor=factor))
+    p = rcv.traverse(sc)
+    p[KEY] = torch.from_numpy(fine).cuda()
+    p.update()
+    p.update()                                # old = new = fine
+    fresh = rcv.load_dict(volume_box_scene(fine, scale=5.0, w=12, h=10, spp=8, majorant_factor=factor))
+    for integ in ({"type": "prbvolpath", "max_depth": 8}, {"type": "twostateprbvolpath", "max_depth": 8}):
+        it = rcv.load_dict(integ)
+        a = it.render(sc, 0, 3, 8)
+        b = it.render(fresh, 0, 3, 8)
+        assert torch.equal(a, b), integ
+    if factor:
+        assert np.array_equal(sc.majorant_grid(0, 1), fresh.majorant_grid(0, 1))
+        assert sc.majorant_grid(0, 1).shape == (6, 6, 6)
+    # the previous NEW state became OLD, repeated to the new resolution
+    p[KEY] = torch.from_numpy(np.ones((24, 24, 24, 1), np.float32)).cuda()
+    p.update()
+    old = sc.param_state(KEY, 0).cpu().numpy()
+    assert old.shape == (24, 24, 24, 1)
+    assert np.array_equal(old, np.repeat(np.repeat(np.repeat(fine, 2, 0), 2, 1), 2, 2))
+
+
+def test_run_opt_upsampling_schedule_and_checkpoints(cuda):
+    announce("run_opt upsample + schedule")
+    rng = np.random.default_rng(51)
+    target = rng.uniform(0.2, 1.0, (16, 16, 16, 1)).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(np.full((8, 8, 8, 1), 0.5, np.float32), scale=5.0, w=16, h=12, spp=8))
+    base = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 8})
+    cfg = {"scene": sc, "integrator": base, "method": "cv_ps", "loss": "L1", "n_steps": 12, "lr": 0.02,
+           "spp": 8, "spp_ref": 64, "recomp_freq": 4, "spp_inf": 16, "upsample": [0.5], "schedule": [0.75],
+           "use_majorant_supergrid": True,
+           "variables": {KEY: {"init": np.full((8, 8, 8, 1), 0.5, np.float32), "clamp": (0.0, 1e9)}},
+           "target": {KEY: torch.from_numpy(target).cuda()}}
+    # the 8^3 start takes factor 2 (8 // 2 = 4 cells); the 16^3 grid factor 4
+    res = run_opt(cfg)
+    final = res["final"][KEY]
+    assert tuple(final.shape) == (16, 16, 16, 1)
+    assert sc.majorant_resolution_factor(0) == 4
+    assert sc.majorant_grid(0, 1).shape == (4, 4, 4)
+    integ = res["integrator"]
+    # reset after step 6: the CV iteration counter restarted with steps 7..11
+    assert integ.it == 5
+    assert res["loss"].shape == (4,) and np.all(res["loss"][:3] > 0)
+    assert np.isfinite(res["param_mse"]).all()
+    assert bool(torch.isfinite(final).all()) and float(final.min()) >= 0.0
+
+
+def test_resample_nearest_is_voxel_repetition(cuda):
+    x = torch.arange(2 * 3 * 4 * 2, dtype=torch.float32, device="cuda").reshape(2, 3, 4, 2)
+    y = resample_nearest(x, (4, 6, 8, 2)).cpu().numpy()
+    r = np.repeat(np.repeat(np.repeat(x.cpu().numpy(), 2, 0), 2, 1), 2, 2)
+    assert np.array_equal(y, r)
+    t = torch.rand(5, 7, 3, device="cuda")
+    assert np.array_equal(resample_nearest(t, (10, 14, 3)).cpu().numpy(),
+                          np.repeat(np.repeat(t.cpu().numpy(), 2, 0), 2, 1))

@@ -22,7 +22,7 @@ def test_pcg32_known_answer():
 def test_tea_known_answer_and_host_copy():
     # TEA, 32 rounds, key 0, plaintext 0 -> 41ea3a0a 94baa940
     assert R.tea(0, 0, 32, [0, 0, 0, 0]) == (0x41EA3A0A, 0x94BAA940)
-    # the host-side seed_grad derivation (render.py) agrees with the oracle
+    # the host-side seed_grad derivation (rendering.py) agrees with the oracle
     key = [0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E]
     for v0, v1 in [(0, 1), (7, 1), (123456, 99), (0xFFFFFFFF, 5)]:
         assert sample_tea_32(v0, v1) == R.tea(v0, v1, 4, key)
