@@ -146,8 +146,8 @@ __global__ void batch_supergrid_kernel(const BatchJobDev *__restrict__ jobs, con
 }
 
 // ---------------------------------------------------------------- 2. render
-template <int NS, int INTEG>
-__global__ __launch_bounds__(512) void batch_render_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
+template <int NS, int INTEG, int F>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void batch_render_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
                                     const int64_t *__restrict__ ends, int n, int64_t total, unsigned *queue,
                                     int grab) {
   extern __shared__ int lds_stack[];
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(512) void batch_render_kernel(const BatchJobDev *__
           const Ray ray = camera_ray(sen, px, py, rng);
           V3 L[NS];
           if (INTEG == RCV_INTEGRATOR_VOLPATH)
-            volpath_sample<NS, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
+            volpath_sample<NS, false, F>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
           else
             path_sample<NS, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
 #pragma unroll
@@ -294,8 +294,8 @@ __global__ __launch_bounds__(kElemBlock) void batch_zero_grad_kernel(BatchJobDev
   if (i < J.n_param) J.grad[i] = 0.0f;
 }
 
-template <int INTEG>
-__global__ __launch_bounds__(512) void batch_adjoint_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
+template <int INTEG, int F>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void batch_adjoint_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
                                      const int64_t *__restrict__ ends, int n, int64_t total, unsigned *queue,
                                      int grab) {
   extern __shared__ int lds_stack[];
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(512) void batch_adjoint_kernel(const BatchJobDev *_
             const Pcg32 rng0 = rng;
             V3 L[1];
             if (INTEG == RCV_INTEGRATOR_VOLPATH)
-              volpath_sample<1, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
+              volpath_sample<1, false, F>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
             else
               path_sample<1, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
             if (!any_nonzero(L[0])) continue;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(512) void batch_adjoint_kernel(const BatchJobDev *_
             rng = rng0;
             V3 L2[1];
             if (INTEG == RCV_INTEGRATOR_VOLPATH)
-              volpath_sample<1, true>(S, J.integ, rng, ray, L2, stack, cnt, &adj, &J.gt);
+              volpath_sample<1, true, F>(S, J.integ, rng, ray, L2, stack, cnt, &adj, &J.gt);
             else
               path_sample<1, true>(S, J.integ, rng, ray, L2, stack, cnt, &adj, &J.gt);
           }
@@ -415,6 +415,7 @@ struct Batch {
   int32_t *d_sg_ids = nullptr;
   int64_t *d_sg_ends = nullptr;
   int block = 256, grab = 256, grid_render = 0, grid_adj = 0;
+  int feat = 0;  // kFeat* union over the volume jobs' scenes
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   ~Batch() {
@@ -531,6 +532,7 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
       for (int k = 0; k < 3; ++k) J.sg_res[k] = m.sg_res[k];
     }
   RCV_REQUIRE(n_sg <= 1, "rcv_batch_add_job: the optimised grid drives %d media with supergrids (at most 1)", n_sg);
+  if (J.itype == RCV_INTEGRATOR_VOLPATH) b->feat |= s->features();
   b->host_jobs.push_back(J);
   b->scenes.push_back(s);
   ++const_cast<Scene *>(s)->batch_refs;
@@ -657,8 +659,13 @@ extern "C" int rcv_batch_set_launch(rcv_batch *b, int32_t block_size, int32_t qu
   b->grab = queue_grab;
   const size_t lds = size_t(kStackSize) * block_size * sizeof(int);
   int g1 = 0, g2 = 0;
-  if (int rc = persistent_grid(batch_render_kernel<2, RCV_INTEGRATOR_VOLPATH>, block_size, lds, g1)) return rc;
-  if (int rc = persistent_grid(batch_adjoint_kernel<RCV_INTEGRATOR_VOLPATH>, block_size, lds, g2)) return rc;
+  int rc = RCV_OK;
+  with_features(b->feat, [&](auto f) {
+    rc = persistent_grid(batch_render_kernel<2, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>, block_size, lds, g1);
+    if (rc == RCV_OK)
+      rc = persistent_grid(batch_adjoint_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>, block_size, lds, g2);
+  });
+  if (rc != RCV_OK) return rc;
   b->grid_render = g1;
   b->grid_adj = g2;
   if (b->graph_exec) {
@@ -690,17 +697,21 @@ static int enqueue_step(rcv_batch *b, hipStream_t st) {
     const int nc = int(bc.ids.size());
     const dim3 grid(b->grid_render), blk(b->block);
     if (c == 0)
-      hipLaunchKernelGGL((batch_render_kernel<1, RCV_INTEGRATOR_PATH>), grid, blk, lds, st, b->d_jobs, b->d_ids[c],
-                         b->d_ends[c], nc, bc.total, q, b->grab);
+      hipLaunchKernelGGL((batch_render_kernel<1, RCV_INTEGRATOR_PATH, 0>), grid, blk, lds, st, b->d_jobs,
+                         b->d_ids[c], b->d_ends[c], nc, bc.total, q, b->grab);
     else if (c == 1)
-      hipLaunchKernelGGL((batch_render_kernel<2, RCV_INTEGRATOR_PATH>), grid, blk, lds, st, b->d_jobs, b->d_ids[c],
-                         b->d_ends[c], nc, bc.total, q, b->grab);
+      hipLaunchKernelGGL((batch_render_kernel<2, RCV_INTEGRATOR_PATH, 0>), grid, blk, lds, st, b->d_jobs,
+                         b->d_ids[c], b->d_ends[c], nc, bc.total, q, b->grab);
     else if (c == 2)
-      hipLaunchKernelGGL((batch_render_kernel<1, RCV_INTEGRATOR_VOLPATH>), grid, blk, lds, st, b->d_jobs,
-                         b->d_ids[c], b->d_ends[c], nc, bc.total, q, b->grab);
+      with_features(b->feat, [&](auto f) {
+        hipLaunchKernelGGL((batch_render_kernel<1, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), grid, blk, lds, st,
+                           b->d_jobs, b->d_ids[c], b->d_ends[c], nc, bc.total, q, b->grab);
+      });
     else
-      hipLaunchKernelGGL((batch_render_kernel<2, RCV_INTEGRATOR_VOLPATH>), grid, blk, lds, st, b->d_jobs,
-                         b->d_ids[c], b->d_ends[c], nc, bc.total, q, b->grab);
+      with_features(b->feat, [&](auto f) {
+        hipLaunchKernelGGL((batch_render_kernel<2, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), grid, blk, lds, st,
+                           b->d_jobs, b->d_ids[c], b->d_ends[c], nc, bc.total, q, b->grab);
+      });
     if (int rc = check_launch("rcv_batch_step: render")) return rc;
   }
   hipLaunchKernelGGL(batch_combine_loss_kernel, dim3(unsigned(b->elem_total / kElemBlock)), dim3(kElemBlock), 0,
@@ -714,11 +725,14 @@ static int enqueue_step(rcv_batch *b, hipStream_t st) {
     unsigned *q = b->d_queues + 4 + c;
     const int nc = int(bc.ids.size());
     if (c == 0)
-      hipLaunchKernelGGL((batch_adjoint_kernel<RCV_INTEGRATOR_PATH>), dim3(b->grid_adj), dim3(b->block), lds, st,
+      hipLaunchKernelGGL((batch_adjoint_kernel<RCV_INTEGRATOR_PATH, 0>), dim3(b->grid_adj), dim3(b->block), lds, st,
                          b->d_jobs, b->d_ids[4 + c], b->d_ends[4 + c], nc, bc.total, q, b->grab);
     else
-      hipLaunchKernelGGL((batch_adjoint_kernel<RCV_INTEGRATOR_VOLPATH>), dim3(b->grid_adj), dim3(b->block), lds, st,
-                         b->d_jobs, b->d_ids[4 + c], b->d_ends[4 + c], nc, bc.total, q, b->grab);
+      with_features(b->feat, [&](auto f) {
+        hipLaunchKernelGGL((batch_adjoint_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), dim3(b->grid_adj),
+                           dim3(b->block), lds, st, b->d_jobs, b->d_ids[4 + c], b->d_ends[4 + c], nc, bc.total, q,
+                           b->grab);
+      });
     if (int rc = check_launch("rcv_batch_step: adjoint")) return rc;
   }
   hipLaunchKernelGGL(batch_adam_kernel, dim3(unsigned(b->pelem_total / kElemBlock)), dim3(kElemBlock), 0, st,

@@ -3,10 +3,29 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/rcv.h"
 #include "scene_internal.hpp"
 
 namespace rcv {
+
+// Calls fn(std::integral_constant<int, F>{}) for the run-time feature mask
+// f, so a launch site can pick the kernel compiled for exactly the scene's
+// features (kFeat* bits).
+template <typename Fn>
+inline void with_features(int f, Fn &&fn) {
+  switch (f & kFeatAll) {
+    case 0: fn(std::integral_constant<int, 0>{}); break;
+    case 1: fn(std::integral_constant<int, 1>{}); break;
+    case 2: fn(std::integral_constant<int, 2>{}); break;
+    case 3: fn(std::integral_constant<int, 3>{}); break;
+    case 4: fn(std::integral_constant<int, 4>{}); break;
+    case 5: fn(std::integral_constant<int, 5>{}); break;
+    case 6: fn(std::integral_constant<int, 6>{}); break;
+    default: fn(std::integral_constant<int, 7>{}); break;
+  }
+}
 
 // Splits spp camera samples of a pixel over 2^chunk_log2 threads of spt
 // samples each (see render.hip).

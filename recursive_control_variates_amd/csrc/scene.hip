@@ -447,6 +447,17 @@ int Scene::relayout_supergrids(hipStream_t stream) {
   return RCV_OK;
 }
 
+int Scene::features() const {
+  int f = 0;
+  for (const ShapeInfo &sh : shapes)
+    if (sh.bsdf < 0 || bsdfs[sh.bsdf].type != RCV_BSDF_NULL) f |= kFeatSurfaces;
+  for (const MediumInfo &m : media) {
+    if (m.albedo_param >= 0) f |= kFeatAlbedoGrid;
+    if (m.supergrid_offset >= 0) f |= kFeatSupergrid;
+  }
+  return f;
+}
+
 Scene::~Scene() {
   if (dev_block) (void)hipFree(dev_block);
   if (sg_block) (void)hipFree(sg_block);

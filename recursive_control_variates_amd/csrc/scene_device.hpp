@@ -380,7 +380,11 @@ __device__ inline V3 emitter_eval(const SceneView &S, int em, const SurfInt &si,
 
 // Emitter hit by `si` (environment for misses), -1 if none.
 __device__ inline int si_emitter(const SceneView &S, const SurfInt &si) {
-  if (!si.valid) return S.env_emitter;
+  // The environment id is taken as a value (readfirstlane of a uniform
+  // kernel argument): a select between a pointer into the shape table and a
+  // pointer to the argument would copy the argument to scratch memory.
+  const int env = __builtin_amdgcn_readfirstlane(S.env_emitter);
+  if (!si.valid) return env;
   return S.shapes[si.shape].emitter;
 }
 

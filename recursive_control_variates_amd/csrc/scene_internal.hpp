@@ -55,6 +55,7 @@ struct Scene {
   int64_t plan_supergrids();
   int alloc_supergrids();
   int relayout_supergrids(hipStream_t stream);
+  int features() const;  // kFeat* mask of the committed scene
 };
 
 // Decodes an order-preserving key written by the majorant reduction.

@@ -88,6 +88,14 @@ struct SensorInfo {
   ViewInfo views[kMaxViews];
 };
 
+// Scene features a volume kernel is compiled for (its template parameter F):
+// code for an absent feature is left out, which keeps the register count of
+// the common configurations low enough for three waves per SIMD.
+constexpr int kFeatSurfaces = 1;    // a shape whose BSDF is not null (diffuse shading, surface NEE)
+constexpr int kFeatAlbedoGrid = 2;  // a medium whose albedo is a grid
+constexpr int kFeatSupergrid = 4;   // a medium with a majorant supergrid
+constexpr int kFeatAll = 7;
+
 struct ParamBind {
   const float *data[2];  // RCV_STATE_OLD, RCV_STATE_NEW
   int32_t nx, ny, nz, ch;

@@ -189,7 +189,7 @@ __device__ inline float supergrid_free_flight(const SceneView &S, const MediumIn
 // TwoStateMedium.sample_interaction_twostates (plugins/twostatemedium.py:
 // 15-107): one global majorant max(maj_old, maj_new) (:71-78), or the
 // supergrid DDA when the medium has a majorant supergrid (:25-70, 82-85).
-template <int NS, bool FOOT>
+template <int NS, int F>
 __device__ inline void sample_medium(const SceneView &S, int mid, const Ray &ray, float u, MediumEv<NS> &ev) {
   const MediumInfo &M = S.media[mid];
   float t0, t1;
@@ -200,7 +200,7 @@ __device__ inline void sample_medium(const SceneView &S, int mid, const Ray &ray
   const float scale = M.scale;
   const float tau = -logf(1.0f - u);
   float maj = 0.0f, t = kInf;
-  if (M.supergrid_offset >= 0) {
+  if ((F & kFeatSupergrid) && M.supergrid_offset >= 0) {
     if (ok) t = supergrid_free_flight<NS>(S, M, ray, mint, maxt, tau, maj);
     // A collision is kept only where its cell's majorant is positive and
     // finite, the invariant of the single-majorant branch below.
@@ -234,7 +234,7 @@ __device__ inline void sample_medium(const SceneView &S, int mid, const Ray &ray
       const int s = state_index<NS>(k);
       ev.st[k] = scale * grid_eval1(P.data[s], ev.foot);
       V3 alb;
-      if (M.albedo_param >= 0) {
+      if ((F & kFeatAlbedoGrid) && M.albedo_param >= 0) {
         const ParamBind &A = S.params[M.albedo_param];
         GridFoot fa;
         grid_footprint(A, M, ev.p, fa);
@@ -275,7 +275,7 @@ __device__ inline bool null_bsdf(const SceneView &S, int shape) {
 // direction without visibility, then ratio tracking of both transmittances.
 // GRAD = true replays the same random numbers and scatters
 // d(log T)/d(density) * gscale into the density gradient.
-template <int NS, bool GRAD>
+template <int NS, bool GRAD, int F>
 __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, bool ref_surface, int medium,
                                      Pcg32 &rng, V3 em[NS], DirSample &ds, const LdsStack &stack,
                                      LocalCounters &cnt, float gscale, const GradTargets *G) {
@@ -314,7 +314,7 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
     mev.t = kInf;
     if (active_medium) {
       const float u = rng.next_1d();
-      sample_medium<NS, GRAD>(S, medium, ray, u, mev);
+      sample_medium<NS, F>(S, medium, ray, u, mev);
       cnt.c[kCntNeeSteps]++;
       if (si.t < mev.t) {
         mev.valid = false;
@@ -342,7 +342,7 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
     }
     active_surface = (active_surface || escaped) && si.valid && !active_medium;
     if (active_surface) {
-      const float b = null_bsdf(S, si.shape) ? 1.0f : 0.0f;
+      const float b = (!(F & kFeatSurfaces) || null_bsdf(S, si.shape)) ? 1.0f : 0.0f;
 #pragma unroll
       for (int k = 0; k < NS; ++k) trm[k] = trm[k] * b;
     }
@@ -376,7 +376,7 @@ struct AdjState {
 // One camera sample. NS = 2: L[0] = old (H), L[1] = new (F). REPLAY (NS = 1):
 // second pass of path-replay backpropagation; the primal radiance has been
 // stored in adj->Lrem by the caller.
-template <int NS, bool REPLAY>
+template <int NS, bool REPLAY, int F>
 __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, Ray ray, V3 L[NS],
                                const LdsStack &stack, LocalCounters &cnt, AdjState *adj,
                                const GradTargets *G) {
@@ -429,7 +429,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
     mev.t = kInf;
     if (active_medium) {
       const float u = rng.next_1d();
-      sample_medium<NS, REPLAY>(S, medium, ray, u, mev);
+      sample_medium<NS, F>(S, medium, ray, u, mev);
       cnt.c[kCntCollisions]++;
       if (needs_int) {
         Hit h;
@@ -506,7 +506,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
         }
         cnt.c[kCntGradAtomics] += 8;
       }
-      if (act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
+      if ((F & kFeatAlbedoGrid) && act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
         const V3 dl = adj->dL * adj->Lrem;
         const V3 ga = v3(mev.alb.x != 0.f ? dl.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? dl.y / mev.alb.y : 0.f,
                          mev.alb.z != 0.f ? dl.z / mev.alb.z : 0.f);
@@ -563,7 +563,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
     if (active_surface) {
       const int b = S.shapes[si.shape].bsdf;
       if (b >= 0) bsdf = S.bsdfs[b];
-      smooth = bsdf.type == RCV_BSDF_DIFFUSE;
+      smooth = (F & kFeatSurfaces) && bsdf.type == RCV_BSDF_DIFFUSE;
       fr = make_frame(si.n);
       wi = fr.to_local(-ray.d);
     }
@@ -579,7 +579,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
       V3 em[NS];
       DirSample ds;
       const Pcg32 rng_nee = rng;
-      nee_twostates<NS, false>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G);
+      nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G);
       V3 nw[NS];
       float nee_pdf;
       if (active_e_surface) {
@@ -606,7 +606,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
           Pcg32 r2 = rng_nee;
           V3 em2[NS];
           DirSample ds2;
-          nee_twostates<NS, true>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G);
+          nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G);
         }
         adj->Lrem -= c[0];
       }
@@ -628,7 +628,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
         bs.pdf = 1.0f;
         bs.weight = v3(1.f);
         bs.is_null = true;
-      } else if (bsdf.type == RCV_BSDF_DIFFUSE && wi.z > 0.0f) {
+      } else if ((F & kFeatSurfaces) && bsdf.type == RCV_BSDF_DIFFUSE && wi.z > 0.0f) {
         bs.wo = square_to_cosine_hemisphere(a, b);
         bs.pdf = diffuse_pdf(wi, bs.wo);
         rho_new = diffuse_reflectance(S, bsdf, state_index<NS>(NS - 1), si.u, si.v);
@@ -640,7 +640,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
       if (NS == 2) {
         // old state: f_old / pdf unless the sampled lobe is delta (null)
         V3 w_old;
-        if (bs.is_null) {
+        if (bs.is_null || !(F & kFeatSurfaces)) {
           w_old = bs.weight;
         } else {
           const V3 f_old = diffuse_eval(diffuse_reflectance(S, bsdf, RCV_STATE_OLD, si.u, si.v), wi, bs.wo);
