@@ -70,7 +70,7 @@ typedef struct rcv_integrator_desc {
                                  per thread (1..spp, divides spp); 0 = auto.
                                  Does not change results beyond float
                                  summation order of the pixel average. */
-  int32_t block_size;    /* threads per block, 64..1024 multiple of 64; 0 = auto */
+  int32_t block_size;    /* threads per block, 64..256 multiple of 64; 0 = 256 */
 } rcv_integrator_desc;
 
 /* Heterogeneous medium with an axis-aligned grid box (mitsuba 'heterogeneous'

@@ -252,7 +252,7 @@ class BatchOptimizer:
             pass
 
 
-def autotune(make_specs, blocks=(64, 128, 256, 512), grabs=(64, 256, 1024, 4096), steps=3, device="cuda"):
+def autotune(make_specs, blocks=(64, 128, 256, 512), grabs=(64, 128, 256, 1024), steps=3, device="cuda"):
     """Times ``steps`` batched iterations for every (block size, queue grab)
     pair on a throw-away batch built by ``make_specs()`` and returns the
     fastest pair with the full table of timings (ms per iteration)."""

@@ -41,6 +41,7 @@ namespace rcv {
 
 constexpr int kWave = 64;
 constexpr int kElemBlock = 256;
+constexpr int kBatchMaxSpt = 8;  // camera samples per work unit (at most)
 
 struct BatchJobDev {
   SceneView view;       // params[param].data[] rotated on the device
@@ -176,8 +177,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
       const uint32_t seed = iter_seed(J);
       if (pixel < J.n_pix) {
         const int px = int(pixel % J.film_w), py = int(pixel / J.film_w);
-        for (int s_ = 0; s_ < J.spt; ++s_) {
-          const int s = chunk * J.spt + s_;
+        const int s_end = sample_end(chunk, J.spp, J.cl2);
+        for (int s = sample_end(chunk - 1, J.spp, J.cl2); s < s_end; ++s) {
           Pcg32 rng = lane_rng(seed, uint32_t(pixel * J.spp + s));
           const Ray ray = camera_ray(sen, px, py, rng);
           V3 L[NS];
@@ -325,8 +326,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
                          J.grad_img[3 * pixel + 2] / inv);
         const uint32_t seed = iter_seed_grad(J);
         if (any_nonzero(dL)) {
-          for (int s_ = 0; s_ < J.spt_g; ++s_) {
-            const int s = chunk * J.spt_g + s_;
+          const int s_end = sample_end(chunk, J.spp_g, J.cl2_g);
+          for (int s = sample_end(chunk - 1, J.spp_g, J.cl2_g); s < s_end; ++s) {
             Pcg32 rng = lane_rng(seed, uint32_t(pixel * J.spp_g + s));
             const Ray ray = camera_ray(sen, px, py, rng);
             const Pcg32 rng0 = rng;
@@ -478,8 +479,10 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
               "rcv_batch_add_job: pixels * spp must stay below 2^32");
   J.spp = d->spp;
   J.spp_g = d->spp_grad;
-  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp, J.spt, J.cl2)) return rc;
-  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp_g, J.spt_g, J.cl2_g)) return rc;
+  // finer work units than a single launch: a batch has parallelism to
+  // spare, and short units keep the end of a persistent launch balanced
+  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp, J.spt, J.cl2, kBatchMaxSpt)) return rc;
+  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp_g, J.spt_g, J.cl2_g, kBatchMaxSpt)) return rc;
   J.seed = d->seed;
   J.param = d->param;
   J.n_param = int64_t(P.nx) * P.ny * P.nz * P.ch;
@@ -650,7 +653,7 @@ static int persistent_grid(K kernel, int block, size_t lds, int &grid) {
 extern "C" int rcv_batch_set_launch(rcv_batch *b, int32_t block_size, int32_t queue_grab) {
   RCV_REQUIRE(b && b->committed, "rcv_batch_set_launch: batch not committed");
   if (block_size == 0) block_size = 256;
-  if (queue_grab == 0) queue_grab = 256;
+  if (queue_grab == 0) queue_grab = 64;
   RCV_REQUIRE(block_size >= 64 && block_size <= 512 && block_size % 64 == 0,
               "rcv_batch_set_launch: block size must be a multiple of 64 in [64, 512]");
   RCV_REQUIRE(queue_grab >= 64 && queue_grab <= 8192 && queue_grab % 64 == 0,

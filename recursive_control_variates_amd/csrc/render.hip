@@ -1,10 +1,11 @@
 // Render entry points of librcv: plain, two-state and adjoint launches of the
 // surface (path.hpp) and volume (volpath.hpp) integrators.
 //
-// Launch shape: one thread traces `spt` camera samples of one pixel in
-// sequence (samples_per_thread); the spp/spt threads of a pixel sit in the
-// same wavefront and their sums are combined with cross-lane shuffles, so the
-// film needs no atomics and the pixel average (box filter develop, plugins/
+// Launch shape: the spp camera samples of a pixel are split over 2^chunk_log2
+// threads (chunk c traces samples [c spp >> chunk_log2, (c + 1) spp >>
+// chunk_log2) in sequence); the threads of a pixel sit in the same wavefront
+// and their sums are combined with cross-lane shuffles, so the film needs no
+// atomics and the pixel average (box filter develop, plugins/
 // twostatepath.py:168-177) is written once, coalesced. The random stream of a
 // camera sample depends only on (seed, pixel * spp + sample), so the launch
 // shape changes results only through the float summation order of a pixel.
@@ -21,7 +22,8 @@
 namespace rcv {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kMaxBlock = 256;      // launch bound of the render kernels
+constexpr int kDefaultBlock = 256;  // threads per block when the descriptor says 0
 
 __device__ inline void flush_counters(const SceneView &S, LocalCounters &cnt) {
   if (!S.counters) return;
@@ -37,11 +39,11 @@ __device__ inline void flush_counters(const SceneView &S, LocalCounters &cnt) {
 // Three waves per SIMD: measured faster than two without spills or four with
 // them for the volume trackers, which are bound by memory latency.
 template <int NS, int INTEG, int F>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void render_kernel(
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) void render_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
                                                         int spp, int spt, int chunk_log2, float *img_old,
                                                         float *img_new) {
-  __shared__ int lds_stack[kStackSize * kBlock];
+  extern __shared__ int lds_stack[];  // kStackSize ints per thread
   const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
@@ -56,8 +58,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   cnt.zero();
   if (pixel < n_pix) {
     const int px = int(pixel % film_w), py = int(pixel / film_w);
-    for (int j = 0; j < spt; ++j) {
-      const int s = chunk * spt + j;
+    const int s_end = sample_end(chunk, spp, chunk_log2);
+    for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s) {
       Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
       const Ray ray = camera_ray(sen, px, py, rng);
       V3 L[NS];
@@ -97,11 +99,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // same random numbers that scatters d(log weight)/d(theta) times the radiance
 // still to come into the parameter gradients. dimage/dL = 1/spp (box filter).
 template <int INTEG, int F>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void adjoint_kernel(
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) void adjoint_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
                                                          int spp, int spt, int chunk_log2,
                                                          const float *__restrict__ grad_img, GradTargets G) {
-  __shared__ int lds_stack[kStackSize * kBlock];
+  extern __shared__ int lds_stack[];  // kStackSize ints per thread
   const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
@@ -116,8 +118,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const float inv = float(spp);
     const V3 dL = v3(grad_img[3 * pixel] / inv, grad_img[3 * pixel + 1] / inv, grad_img[3 * pixel + 2] / inv);
     if (any_nonzero(dL)) {
-      for (int j = 0; j < spt; ++j) {
-        const int s = chunk * spt + j;
+      const int s_end = sample_end(chunk, spp, chunk_log2);
+      for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s) {
         Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
         const Ray ray = camera_ray(sen, px, py, rng);
         const Pcg32 rng0 = rng;
@@ -144,13 +146,17 @@ __global__ void zero_counters_kernel(unsigned long long *c) {
   if (threadIdx.x < kNumCounters) c[threadIdx.x] = 0ull;
 }
 
-int pow2_part(int v) { return v & -v; }
 
 }  // namespace
 
-int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2) {
+static size_t stack_bytes(int block) { return size_t(kStackSize) * block * sizeof(int); }
+
+int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2, int max_spt) {
+  if (d && d->block_size != 0 && (d->block_size < 64 || d->block_size > kMaxBlock || d->block_size % 64 != 0))
+    return fail(RCV_E_INVALID, "block_size %d must be 0 or a multiple of 64 in [64, %d]", d->block_size, kMaxBlock);
   int chunks;
-  const int cap = std::min(64, pow2_part(spp));
+  int cap = 1;
+  while (cap < 64 && 2 * cap <= spp) cap <<= 1;  // largest power of two <= min(spp, 64)
   if (d && d->samples_per_thread > 0) {
     if (spp % d->samples_per_thread != 0)
       return fail(RCV_E_INVALID, "samples_per_thread %d must divide spp %d", d->samples_per_thread, spp);
@@ -159,9 +165,10 @@ int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, 
       return fail(RCV_E_INVALID, "spp / samples_per_thread = %d must be a power of two <= 64", chunks);
   } else {
     chunks = 1;
-    while (chunks < cap && n_pix * chunks < (int64_t(1) << 19)) chunks <<= 1;
+    while (chunks < cap && (n_pix * chunks < (int64_t(1) << 19) || (max_spt > 0 && spp > max_spt * chunks)))
+      chunks <<= 1;
   }
-  spt = spp / chunks;
+  spt = (spp + chunks - 1) / chunks;  // samples of the busiest thread
   chunk_log2 = 0;
   while ((1 << chunk_log2) < chunks) ++chunk_log2;
   return RCV_OK;
@@ -187,6 +194,7 @@ static int check_states(const Scene *s, int ns, const char *fn) {
 
 struct RenderLaunch {
   dim3 grid;
+  int block;
   hipStream_t stream;
   const SceneView *view;
   IntegArgs I;
@@ -198,7 +206,8 @@ struct RenderLaunch {
 
 template <int NS, int INTEG, int F>
 static void launch_render_kernel(const RenderLaunch &L) {
-  hipLaunchKernelGGL((render_kernel<NS, INTEG, F>), L.grid, dim3(kBlock), 0, L.stream, *L.view, L.I, L.sensor,
+  hipLaunchKernelGGL((render_kernel<NS, INTEG, F>), L.grid, dim3(L.block), stack_bytes(L.block), L.stream, *L.view,
+                     L.I, L.sensor,
                      L.seed, L.spp, L.spt, L.cl2, L.img_old, L.img_new);
 }
 
@@ -218,9 +227,10 @@ int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint
   if (int rc = plan_launch(d, n_pix, spp, spt, cl2)) return rc;
   const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters};
   const int64_t threads = n_pix << cl2;
-  const unsigned grid = ceil_div(threads, kBlock);
+  const int block = d->block_size ? d->block_size : kDefaultBlock;
+  const unsigned grid = ceil_div(threads, block);
   if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
-  const RenderLaunch L{dim3(grid), stream, &s->view, I, sensor, seed, spp, spt, cl2, img_old, img_new};
+  const RenderLaunch L{dim3(grid), block, stream, &s->view, I, sensor, seed, spp, spt, cl2, img_old, img_new};
   if (d->type == RCV_INTEGRATOR_VOLPATH) {
     if (ns == 2)
       with_features(s->features(), [&](auto f) { launch_render_kernel<2, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>(L); });
@@ -258,16 +268,17 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   int spt, cl2;
   if (int rc = plan_launch(d, n_pix, spp, spt, cl2)) return rc;
   const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters};
-  const unsigned grid = ceil_div(n_pix << cl2, kBlock);
+  const int block = d->block_size ? d->block_size : kDefaultBlock;
+  const unsigned grid = ceil_div(n_pix << cl2, block);
   if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
   if (d->type == RCV_INTEGRATOR_VOLPATH)
     with_features(s->features(), [&](auto f) {
-      hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), dim3(grid), dim3(kBlock), 0,
-                         stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);
+      hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), dim3(grid), dim3(block),
+                         stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);
     });
   else
-    hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_PATH, 0>), dim3(grid), dim3(kBlock), 0, stream, s->view, I,
-                       sensor, seed, spp, spt, cl2, grad_img, G);
+    hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_PATH, 0>), dim3(grid), dim3(block), stack_bytes(block), stream,
+                       s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);
   return check_launch(fn);
 }
 

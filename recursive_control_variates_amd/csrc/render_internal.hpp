@@ -27,9 +27,20 @@ inline void with_features(int f, Fn &&fn) {
   }
 }
 
-// Splits spp camera samples of a pixel over 2^chunk_log2 threads of spt
-// samples each (see render.hip).
-int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2);
+// Splits the spp camera samples of a pixel over 2^chunk_log2 threads (at
+// most spt samples each): a user-given samples_per_thread must divide spp
+// into a power of two <= 64 chunks; otherwise chunks double, up to the
+// largest power of two <= min(spp, 64), while the launch has fewer than 2^19
+// threads or (max_spt > 0) a thread would trace more than max_spt samples.
+int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2,
+                int max_spt = 0);
+
+// One past the last sample of chunk c (chunk -1 ends at 0): chunk c traces
+// samples [sample_end(c - 1), sample_end(c)), an even split when the chunk
+// count divides spp.
+__host__ __device__ inline int sample_end(int c, int spp, int chunk_log2) {
+  return int((int64_t(c + 1) * spp) >> chunk_log2);
+}
 
 int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint32_t seed, int spp,
                   float *img_old, float *img_new, int ns, hipStream_t stream, const char *fn);

@@ -10,8 +10,9 @@
 
 Props and defaults follow the reference plugin docstrings: ``max_depth`` 6
 (-1 = unbounded), ``rr_depth`` 5, ``hide_emitters`` False, ``xyz`` False
-(plugins/twostatepath.py:106-118,149). ``samples_per_thread`` is this build's
-launch-shape knob (0 = automatic).
+(plugins/twostatepath.py:106-118,149). ``samples_per_thread`` and
+``block_size`` are this build's launch-shape knobs (0 = automatic); they
+change results only through the float summation order of a pixel.
 """
 from __future__ import annotations
 
@@ -43,6 +44,7 @@ class RenderIntegrator:
         self.rr_depth = int(p.get("rr_depth", 5))
         self.hide_emitters = bool(p.get("hide_emitters", False))
         self.samples_per_thread = int(p.get("samples_per_thread", 0))
+        self.block_size = int(p.get("block_size", 0))
         self.xyz = bool(p.get("xyz", False))
         if self.max_depth < -1 or self.max_depth == 0:
             raise ValueError("max_depth must be -1 or positive")
@@ -54,7 +56,7 @@ class RenderIntegrator:
         d.rr_depth = self.rr_depth
         d.hide_emitters = int(self.hide_emitters)
         d.samples_per_thread = self.samples_per_thread
-        d.block_size = 0
+        d.block_size = self.block_size
         return d
 
     @staticmethod

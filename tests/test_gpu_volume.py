@@ -100,9 +100,9 @@ def test_counters_and_launch_shapes(cuda):
     sc = rcv.load_dict(volume_box_scene(g, scale=5.0, w=16, h=16, spp=16))
     sc.set_counting(True)
     imgs = []
-    for spt in (0, 1, 4, 16):
+    for spt, blk in ((0, 0), (1, 64), (4, 128), (16, 256), (0, 64)):
         integ = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 8,
-                               "samples_per_thread": spt})
+                               "samples_per_thread": spt, "block_size": blk})
         H, F = integ.render_twostates(sc, 0, 9, 16)
         c = sc.read_counters()
         assert c["paths"] == 16 * 16 * 16
@@ -129,6 +129,9 @@ def test_scene_checks_and_errors(cuda):
         ts.render_twostates(sc, 3, 1, 4)   # no such sensor
     with pytest.raises(N.RcvError, match="samples_per_thread"):
         bad = rcv.load_dict({"type": "twostateprbvolpath", "samples_per_thread": 3})
+        bad.render_twostates(sc, 0, 1, 4)
+    with pytest.raises(N.RcvError, match="block_size"):
+        bad = rcv.load_dict({"type": "twostateprbvolpath", "block_size": 96})
         bad.render_twostates(sc, 0, 1, 4)
     p = rcv.traverse(sc)
     t = p["box.interior_medium.sigma_t.data"]
