@@ -220,6 +220,37 @@ def kernel_roofline(args, st):
     return out
 
 
+def supergrid_compare(args, st, factor=8):
+    """The step's two kernels at the current parameter state with a majorant
+    supergrid of factor ``factor`` (ceil(grid / factor)^3 cells, DDA free
+    flights) against the single global majorant: event-timed durations and
+    tentative-collision counts from the device counters. Same samples, so the
+    difference is the tracker's work."""
+    import torch
+    sc, integ = st.sc, st.base
+    n_pix = args.res * args.res * 4
+    gimg = torch.full(sc.film_shape(0), 1.0 / (n_pix * 3), device="cuda")
+    st.params.update(st.opt)
+    out = {}
+    for f in (0, factor):
+        sc.set_majorant_factor(0, f)
+        integ.render_twostates(sc, 0, 999, args.spp)
+        torch.cuda.synchronize()
+        r_ms = _event_time(lambda r: integ.render_twostates(sc, 0, 3000 + r, args.spp), args.kernel_reps)
+        a_ms = _event_time(lambda r: integ.adjoint(sc, st.params, gimg, 0, 4000 + r, args.spp), args.kernel_reps)
+        sc.set_counting(True)
+        integ.render_twostates(sc, 0, 3000, args.spp)
+        rc = sc.read_counters()
+        integ.adjoint(sc, st.params, gimg, 0, 4000, args.spp)
+        ac = sc.read_counters()
+        sc.set_counting(False)
+        out[f"factor_{f}"] = {"twostate_ms": round(r_ms, 3), "adjoint_ms": round(a_ms, 3),
+                              "twostate_collisions": rc["collisions"], "twostate_null_events": rc["null_events"],
+                              "adjoint_collisions": ac["collisions"], "adjoint_nee_steps": ac["nee_steps"]}
+    sc.set_majorant_factor(0, 0)
+    return out
+
+
 MSE_LEVELS = (1e-3, 2e-2, 1.5e-2, 1.25e-2)
 
 
@@ -417,6 +448,8 @@ def main():
     log(f"[bench] {args.steps} steps: {ms:.3f} ms/step, {msps:.1f} Msamples/s")
     roof = kernel_roofline(args, st)
     log(f"[bench] roofline {roof}")
+    sg = supergrid_compare(args, st)
+    log(f"[bench] supergrid {sg}")
     img_ref = st.img_ref
     del st
     ttm = time_to_mse(args, img_ref) if args.mse_steps > 0 else None
@@ -437,6 +470,7 @@ def main():
         "roofline": {k: v for k, v in roof["volume_adjoint"].items()
                      if k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_detail": roof,
+        "majorant_supergrid": sg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -9,19 +9,31 @@ clamping, resolution doubling at the ``upsample`` fractions (with the CV
 reset and the supergrid factor update), learning-rate halving at the
 ``schedule`` fractions.
 
-Extras for measurement: if ``target`` ({key: tensor}) is given, the
-parameter MSE against it is recorded every step; ``benchmark`` records the
-wall-clock time of every step (GPU synchronised), excluding nothing;
-statistics skip the first 17 steps as the reference does (optimize.py:105).
+With ``output_dir`` the run reads and writes the reference's files: the
+reference image ``img_ref.exr`` and start image ``img_start.exr`` next to
+``output_dir`` (reused when present, utils.py:98-111, optimize.py:91-97),
+checkpoint images with ``save`` (``img_inf``, ``img``, ``weights``,
+``img_H``, ``img_F`` as EXR + PNG, utils.py:129-156), ``img_final.exr`` and
+the final parameters (``.vol`` grids, ``.exr`` textures, ``final_state.json``
+for other values, utils.py:45-60).
+
+``result["runtime"]`` holds, for steps i > 16 with ``benchmark``, the GPU-
+synchronised [primal + loss, backward + step] milliseconds
+(optimize.py:47,105-132); ``step_ms`` the same two phases summed for every
+step. Extras for measurement: if ``target`` ({key: tensor}) is given, the
+parameter MSE against it is recorded every step (the check synchronises).
 """
 from __future__ import annotations
 
+import json
+import os
 import time
 from typing import Dict
 
 import numpy as np
 import torch
 
+from . import fileio
 from .cv_integrator import MetaIntegrator
 from .optim import Adam, l1_loss, l2_loss
 from .rendering import render
@@ -38,7 +50,14 @@ def _loss_func(name):
 
 def render_reference(cfg, scene, integrator, params=None, target=None):
     """Average of ``ref_passes`` renders at ``spp_ref`` with seeds 17843 + j
-    (utils.py:98-111), of the target parameter values if given."""
+    (utils.py:98-111), of the target parameter values if given. With
+    ``output_dir``, ``img_ref.exr`` beside it is read if present, else
+    written."""
+    save_path = None
+    if cfg.get("output_dir"):
+        save_path = os.path.join(os.path.dirname(os.path.abspath(cfg["output_dir"])), "img_ref.exr")
+        if os.path.isfile(save_path):
+            return torch.from_numpy(fileio.read_exr_rgb(save_path)).to(scene.device)
     ref_passes = int(cfg.get("ref_passes", 1))
     spp_ref = int(cfg.get("spp_ref", cfg.get("spp", 1)))
     saved = {}
@@ -58,7 +77,53 @@ def render_reference(cfg, scene, integrator, params=None, target=None):
             params[k] = v
         params.update()
         params.update()
+    if save_path:
+        fileio.write_exr(save_path, img)
     return img
+
+
+def save_final_state(cfg, opt, output_dir):
+    """utils.py:45-60: grids as .vol, textures as .exr, anything else in
+    final_state.json."""
+    final_state = {}
+    for key in cfg["variables"].keys():
+        x = opt[key].detach()
+        name = key.replace(".", "_") + "_final"
+        if x.dim() == 4:
+            fileio.write_vol(os.path.join(output_dir, name + ".vol"), x)
+        elif x.dim() == 3:
+            fileio.write_exr(os.path.join(output_dir, name + ".exr"), x)
+        else:
+            final_state[key] = x.cpu().numpy().tolist()
+    if final_state:
+        with open(os.path.join(output_dir, "final_state.json"), "w") as f:
+            json.dump(final_state, f)
+
+
+def _checkpoint(cfg, i, integrator, base, scene, loss_func, img, img_ref, result):
+    """utils.py:129-156: every recomp_freq steps, the loss of a fresh plain
+    render at spp_inf (seed i + 1) and the variance of the step's image
+    against it; with ``save``, the images."""
+    recomp_freq = int(cfg.get("recomp_freq", 10))
+    if recomp_freq <= 0 or i % recomp_freq != 0:
+        return
+    out_dir = cfg.get("output_dir")
+    save = bool(cfg.get("save", False)) and out_dir
+    with torch.no_grad():
+        img_inf = base.render(scene, 0, i + 1, int(cfg.get("spp_inf", 64)))
+        result["loss"][i // recomp_freq] = float(loss_func(img_ref, img_inf))
+        result["var"][i // recomp_freq] = float(((img_inf - img.detach()) ** 2).mean())
+        if save:
+            k = i // recomp_freq
+            fileio.save_img(img_inf, out_dir, "img_inf", k, save_png=True)
+            if integrator.img is not None:
+                fileio.save_img(integrator.img, out_dir, "img", k, save_png=True)
+            method = cfg.get("method", "baseline")
+            stats = getattr(integrator, "stats", None)
+            if method != "baseline" and stats is not None and int(stats.n.item()) > integrator.warmup:
+                fileio.save_img(integrator.w_s, out_dir, "weights", k)
+                fileio.save_img(integrator.H, out_dir, "img_H", k)
+                fileio.save_img(integrator.F, out_dir, "img_F", k)
 
 
 def _single_medium(scene) -> int:
@@ -128,7 +193,6 @@ def run_opt(cfg: Dict) -> Dict:
     spp = int(cfg.get("spp", 1))
     spp_grad = int(cfg.get("spp_grad", spp))
     recomp_freq = int(cfg.get("recomp_freq", 10))
-    spp_inf = int(cfg.get("spp_inf", 64))
     benchmark = bool(cfg.get("benchmark", False))
     target = cfg.get("target")
     mse_threshold = cfg.get("mse_threshold")
@@ -158,37 +222,49 @@ def run_opt(cfg: Dict) -> Dict:
         schedule_steps.append(int(n_steps * w))
     result = {"loss": np.zeros(1 + n_steps // max(1, recomp_freq), np.float32),
               "var": np.zeros(1 + n_steps // max(1, recomp_freq), np.float32),
+              "runtime": np.zeros((n_steps, 2), np.float32),
               "step_ms": np.zeros(n_steps, np.float64),
               "param_mse": np.full(n_steps, np.nan),
               "wall_s": np.zeros(n_steps, np.float64),
               "time_to_threshold_s": None, "steps_to_threshold": None}
 
+    out_dir = cfg.get("output_dir")
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
     opt = Adam(lr=lr)
     initialize_parameters(cfg, opt, params)
-    torch.cuda.synchronize(scene.device)
+    if out_dir:
+        start_path = os.path.join(os.path.dirname(os.path.abspath(out_dir)), "img_start.exr")
+        if not os.path.isfile(start_path):
+            with torch.no_grad():
+                fileio.write_exr(start_path, base.render(scene, 0, 2048, int(cfg.get("spp_ref", spp))))
+    dev = scene.device
+    torch.cuda.synchronize(dev)
     t_start = time.perf_counter()
     for i in range(n_steps):
         t0 = time.perf_counter()
         params.update(opt)
         img = render(scene, params, integrator=integrator, seed=i, spp=spp, spp_grad=spp_grad)
         loss = loss_func(img, img_ref)
-        if recomp_freq > 0 and i % recomp_freq == 0 and not benchmark:
-            with torch.no_grad():
-                img_inf = base.render(scene, 0, i + 1, spp_inf)
-                result["loss"][i // recomp_freq] = float(loss_func(img_ref, img_inf))
-                result["var"][i // recomp_freq] = float(((img_inf - img.detach()) ** 2).mean())
+        if benchmark:
+            torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        _checkpoint(cfg, i, integrator, base, scene, loss_func, img, img_ref, result)
+        t2 = time.perf_counter()
         loss.backward()
         opt.step()
+        if benchmark or target is not None:
+            torch.cuda.synchronize(dev)
+        t3 = time.perf_counter()
+        if benchmark and i > 16:
+            result["runtime"][i] = ((t1 - t0) * 1e3, (t3 - t2) * 1e3)
         if i in upsample_steps:
             upsample(scene, cfg, opt, integrator)
         if i in schedule_steps:
             lr *= 0.5
             opt.set_learning_rate(lr)
-        if benchmark or target is not None:
-            torch.cuda.synchronize(scene.device)
-        t1 = time.perf_counter()
-        result["step_ms"][i] = (t1 - t0) * 1e3
-        result["wall_s"][i] = t1 - t_start
+        result["step_ms"][i] = (t1 - t0 + t3 - t2) * 1e3
+        result["wall_s"][i] = t3 - t_start
         if target is not None:
             mse = 0.0
             cnt = 0
@@ -205,6 +281,10 @@ def run_opt(cfg: Dict) -> Dict:
                 result["time_to_threshold_s"] = t1 - t_start
                 result["steps_to_threshold"] = i + 1
     params.update(opt)
+    if out_dir:
+        fileio.write_exr(os.path.join(out_dir, "img_final.exr"),
+                         render_reference(dict(cfg, output_dir=None), scene, base))
+        save_final_state(cfg, opt, out_dir)
     result["final"] = {k: opt[k].detach() for k in opt.keys()}
     result["integrator"] = integrator
     return result

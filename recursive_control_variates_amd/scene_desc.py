@@ -8,10 +8,13 @@ build implements:
   outward normals), ``sphere`` (``center``/``radius``), ``mesh`` (explicit
   ``vertices``/``faces``/optional ``uv``); each with ``to_world``, ``bsdf``,
   ``emitter``, ``interior``/``exterior`` media;
-* BSDFs: ``diffuse`` (``reflectance`` rgb / float / ``bitmap`` with ``data``),
-  ``null``, and the ``twostate`` adapter (plugins/twostatebsdf.py:4-56);
-* media: ``heterogeneous`` with a ``gridvolume`` ``sigma_t`` (1 channel) and
-  an albedo (float / rgb / 3-channel ``gridvolume``), ``scale``,
+* BSDFs: ``diffuse`` (``reflectance`` rgb / float / ``bitmap`` with ``data``
+  or an OpenEXR ``filename``), ``null``, and the ``twostate`` adapter
+  (plugins/twostatebsdf.py:4-56);
+* media: ``heterogeneous`` with a ``gridvolume`` ``sigma_t`` (1 channel;
+  ``data`` or a Mitsuba ``.vol`` ``filename``, placed by ``to_world`` or,
+  with ``use_grid_bbox``, by the file's box) and an albedo (float / rgb /
+  3-channel ``gridvolume``), ``scale``,
   ``sample_emitters``, ``majorant_resolution_factor``; and the
   ``twostatemedium`` adapter (plugins/twostatemedium.py:5-200);
 * emitters: ``area`` (on a shape), ``constant`` (environment);
@@ -151,6 +154,41 @@ class SceneDescription:
         return np.asarray(sp, np.float32).reshape(-1, 4), np.asarray(ss, np.int32)
 
 
+def _grid_data(d, prefix):
+    """(grid, file bbox or None) of a gridvolume: inline ``data`` or a
+    ``filename`` of a float32 Mitsuba .vol file; the file's box is used only
+    with ``use_grid_bbox`` (experiments.py sets it to False and places the
+    grid with ``to_world``)."""
+    if "data" in d:
+        return d["data"], None
+    if "filename" in d:
+        from .fileio import read_vol
+        arr, lo, hi = read_vol(d["filename"])
+        return arr, ((lo, hi) if d.get("use_grid_bbox", False) else None)
+    raise ValueError(f"{prefix}: gridvolume needs 'data' or 'filename'")
+
+
+def _bitmap_data(d, prefix):
+    """Texels of a bitmap: inline ``data`` or an OpenEXR ``filename``."""
+    if "data" in d:
+        return d["data"]
+    if "filename" in d:
+        fn = d["filename"]
+        if not fn.lower().endswith(".exr"):
+            raise ValueError(f"{prefix}: bitmap files must be OpenEXR (.exr)")
+        from .fileio import read_exr_rgb
+        return read_exr_rgb(fn)
+    raise ValueError(f"{prefix}: bitmap needs 'data' or 'filename'")
+
+
+def _box_matrix(lo, hi):
+    m = np.eye(4)
+    for k in range(3):
+        m[k, k] = hi[k] - lo[k]
+        m[k, 3] = lo[k]
+    return m
+
+
 def _as_rgb(v) -> tuple:
     if isinstance(v, dict):
         t = v.get("type")
@@ -244,9 +282,8 @@ class _Loader:
             tex = -1
             rgb = (0.0, 0.0, 0.0)
             if isinstance(refl, dict) and refl.get("type") == "bitmap":
-                if "data" not in refl:
-                    raise ValueError(f"{prefix}: bitmap reflectance needs 'data' (image files are not read)")
-                tex = self._add_param(f"{prefix}.reflectance.data", PARAM_TEXTURE, refl["data"], twostate)
+                tex = self._add_param(f"{prefix}.reflectance.data", PARAM_TEXTURE, _bitmap_data(refl, prefix),
+                                      twostate)
                 if self.desc.params[tex].shape[2] != 3:
                     raise ValueError(f"{prefix}: reflectance bitmap must have 3 channels")
             else:
@@ -275,24 +312,26 @@ class _Loader:
         if phase.get("type") != "isotropic":
             raise ValueError(f"{prefix}: only the isotropic phase function is supported")
         sig = d.get("sigma_t")
-        if not (isinstance(sig, dict) and sig.get("type") == "gridvolume" and "data" in sig):
-            raise ValueError(f"{prefix}: sigma_t must be a gridvolume with 'data'")
-        tw = _transform(sig).matrix
+        if not (isinstance(sig, dict) and sig.get("type") == "gridvolume"):
+            raise ValueError(f"{prefix}: sigma_t must be a gridvolume")
+        sig_data, sig_box = _grid_data(sig, prefix)
+        tw = _transform(sig).matrix if sig_box is None else _box_matrix(*sig_box)
         if np.any(np.abs(tw[:3, :3] - np.diag(np.diag(tw[:3, :3]))) > 1e-12) or np.any(np.diag(tw[:3, :3]) <= 0):
             raise ValueError(f"{prefix}: grid to_world must be an axis-aligned positive scale + translation")
         lo = tw[:3, 3]
         hi = lo + np.diag(tw[:3, :3])
-        sp = self._add_param(f"{prefix}.sigma_t.data", PARAM_GRID, sig["data"], twostate)
+        sp = self._add_param(f"{prefix}.sigma_t.data", PARAM_GRID, sig_data, twostate)
         if self.desc.params[sp].shape[3] != 1:
             raise ValueError(f"{prefix}: sigma_t grid must have 1 channel")
         alb = d.get("albedo", 0.75)
         ap = -1
         argb = (0.0, 0.0, 0.0)
         if isinstance(alb, dict) and alb.get("type") == "gridvolume":
-            ap = self._add_param(f"{prefix}.albedo.data", PARAM_GRID, alb["data"], twostate)
+            alb_data, alb_box = _grid_data(alb, prefix)
+            ap = self._add_param(f"{prefix}.albedo.data", PARAM_GRID, alb_data, twostate)
             if self.desc.params[ap].shape[3] != 3:
                 raise ValueError(f"{prefix}: albedo grid must have 3 channels")
-            aw = _transform(alb).matrix
+            aw = _transform(alb).matrix if alb_box is None else _box_matrix(*alb_box)
             if not np.allclose(aw, tw):
                 raise ValueError(f"{prefix}: albedo and sigma_t grids must share to_world")
         else:

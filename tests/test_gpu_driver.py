@@ -2,6 +2,8 @@
 parameter resolution changes through the scene, upsampling with the CV
 reset and the supergrid factor, the learning-rate schedule and the
 checkpoint arrays."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -79,3 +81,41 @@ def test_resample_nearest_is_voxel_repetition(cuda):
     t = torch.rand(5, 7, 3, device="cuda")
     assert np.array_equal(resample_nearest(t, (10, 14, 3)).cpu().numpy(),
                           np.repeat(np.repeat(t.cpu().numpy(), 2, 0), 2, 1))
+
+
+def test_run_opt_writes_and_reuses_files(cuda, tmp_path):
+    # output_dir: img_ref.exr (reused by a second run), img_start.exr,
+    # checkpoint images, img_final.exr, the final grid as .vol; runtime rows
+    # for steps > 16 with benchmark
+    announce("run_opt files")
+    from recursive_control_variates_amd import fileio
+    rng = np.random.default_rng(52)
+    target = rng.uniform(0.2, 1.0, (6, 6, 6, 1)).astype(np.float32)
+    init = np.full((6, 6, 6, 1), 0.5, np.float32)
+
+    def cfg_for(method, out):
+        sc = rcv.load_dict(volume_box_scene(init, scale=5.0, w=12, h=10, spp=4, twostate=(method == "cv_ps")))
+        base = rcv.load_dict({"type": "twostateprbvolpath" if method == "cv_ps" else "prbvolpath", "max_depth": 6})
+        return {"scene": sc, "integrator": base, "method": method, "n_steps": 20, "lr": 0.02, "spp": 4,
+                "spp_ref": 32, "recomp_freq": 10, "spp_inf": 8, "save": True, "benchmark": True,
+                "output_dir": str(out), "variables": {KEY: {"init": init, "clamp": (0.0, 1e9)}},
+                "target": {KEY: torch.from_numpy(target).cuda()}}
+
+    res = run_opt(cfg_for("cv_ps", tmp_path / "exp" / "cv_ps"))
+    root = tmp_path / "exp"
+    assert (root / "img_ref.exr").is_file() and (root / "img_start.exr").is_file()
+    out = root / "cv_ps"
+    for sub in ("img_inf/0000.exr", "img_inf/0001.png", "img/0001.exr", "weights/0001.exr", "img_H/0001.exr",
+                "img_F/0001.exr", "img_final.exr", KEY.replace(".", "_") + "_final.vol"):
+        assert (out / sub).is_file(), sub
+    grid, lo, hi = fileio.read_vol(str(out / (KEY.replace(".", "_") + "_final.vol")))
+    assert np.array_equal(grid, res["final"][KEY].cpu().numpy())
+    assert np.all(res["runtime"][:17] == 0) and np.all(res["runtime"][17:] > 0)
+    # a second method in the same experiment folder reuses img_ref.exr
+    ref = fileio.read_exr_rgb(str(root / "img_ref.exr"))
+    res2 = run_opt(cfg_for("baseline", root / "baseline"))
+    assert (root / "baseline" / "img_final.exr").is_file()
+    assert np.isfinite(res2["loss"]).all()
+    assert np.array_equal(fileio.read_exr_rgb(str(root / "img_ref.exr")), ref)
+    path = fileio.save_result(str(out), res)
+    assert os.path.isfile(path)
