@@ -52,6 +52,15 @@ extern "C" {
 /* integrator types */
 #define RCV_INTEGRATOR_PATH 0   /* twostateprb / prb  (surfaces)  */
 #define RCV_INTEGRATOR_VOLPATH 1 /* twostateprbvolpath / prbvolpath */
+#define RCV_INTEGRATOR_VOLPATH_SIMPLE 2 /* volpathsimple (plugins/volpathsimple.py):
+                                           one state, DRT adjoint */
+
+/* rcv_integrator_desc.flags for RCV_INTEGRATOR_VOLPATH_SIMPLE (bits set
+ * disable; 0 = the reference defaults, plugins/volpathsimple.py:33-48) */
+#define RCV_SIMPLE_NO_NEE 1
+#define RCV_SIMPLE_NO_DRT 2
+#define RCV_SIMPLE_NO_DRT_SUBSAMPLING 4
+#define RCV_SIMPLE_NO_DRT_MIS 8
 
 /* loss kinds */
 #define RCV_LOSS_L1 0
@@ -71,6 +80,7 @@ typedef struct rcv_integrator_desc {
                                  Does not change results beyond float
                                  summation order of the pixel average. */
   int32_t block_size;    /* threads per block, 64..256 multiple of 64; 0 = 256 */
+  int32_t flags;         /* integrator-specific switches (RCV_SIMPLE_*) */
 } rcv_integrator_desc;
 
 /* Heterogeneous medium with an axis-aligned grid box (mitsuba 'heterogeneous'

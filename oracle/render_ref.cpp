@@ -10,6 +10,7 @@
 //   TwoStateMedium.sample_interaction_twostates        plugins/twostatemedium.py:15-107
 //   TwoStatePRBIntegrator.sample_twostates             plugins/twostatepath.py:225-364
 //   TwoStateBSDF eval/eval_old/sample                  plugins/twostatebsdf.py:25-41
+//   VolpathSimpleIntegrator primal + DRT adjoint       plugins/volpathsimple.py:23-779
 // plus the upstream services they call, with the same conventions as the
 // HIP library (PCG32 with TEA seeding per (seed, pixel*spp+sample), box
 // filter, cell-centred clamped trilinear grids, bilinear repeat textures,
@@ -68,10 +69,11 @@ typedef struct OracleScene {
 } OracleScene;
 
 typedef struct OracleIntegrator {
-  int type;  // 0 path (surfaces), 1 volpath
+  int type;  // 0 path (surfaces), 1 volpath, 2 volpathsimple (NEW state)
   int max_depth;
   int rr_depth;
   int hide_emitters;
+  int flags;  // volpathsimple: 1 no NEE, 2 no DRT, 4 no DRT subsampling, 8 no DRT MIS
 } OracleIntegrator;
 }
 
@@ -1151,6 +1153,319 @@ static void path_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &
   L[1] = L_new;
 }
 
+// ------------------------------------------------------- volpathsimple
+// VolpathSimpleIntegrator (plugins/volpathsimple.py:23-779) on the NEW state
+// (old == new in every call): primal, and the adjoint with differential
+// ratio tracking. The random numbers are drawn in the HIP integrator's order
+// (recursive_control_variates_amd/csrc/volsimple.hpp) so the two can be
+// compared sample by sample. Density gradients only.
+template <typename R>
+struct SimpleAdj {
+  V<R> dL, Lrem;
+  int param = -1;
+  std::vector<double> *g = nullptr;
+  Pcg32 *alt = nullptr;
+};
+
+template <typename R>
+static V<R> albedo_at(const Scene<R> &S, int med, const V<R> &p) {
+  return S.s.med_albedo[med] >= 0 ? S.grid(S.s.med_albedo[med], 1, med, p)
+                                  : V<R>(S.s.med_albedo_rgb[3 * med], S.s.med_albedo_rgb[3 * med + 1],
+                                         S.s.med_albedo_rgb[3 * med + 2]);
+}
+
+template <typename R>
+static bool medium_interval(const Scene<R> &S, int med, const Ray<R> &ray, R &mint, R &maxt) {
+  const float *lo = S.s.med_lo + 3 * med, *hi = S.s.med_hi + 3 * med;
+  R t0 = -std::numeric_limits<R>::infinity(), t1 = std::numeric_limits<R>::infinity();
+  for (int a = 0; a < 3; ++a) {
+    R inv = R(1) / ray.d[a];
+    R ta = (R(lo[a]) - ray.o[a]) * inv, tb = (R(hi[a]) - ray.o[a]) * inv;
+    if (ta > tb) std::swap(ta, tb);
+    t0 = ta > t0 ? ta : t0;
+    t1 = tb < t1 ? tb : t1;
+  }
+  mint = std::max(R(0), t0);
+  maxt = std::min(ray.maxt, t1);
+  return t0 <= t1;
+}
+
+// sample_real_interaction (:315-367)
+template <typename R>
+static MEI<R> simple_real_collision(const Scene<R> &S, int med, Ray<R> ray, Pcg32 &rng) {
+  R run_t = 0;
+  while (true) {
+    MEI<R> mei = sample_interaction_twostates(S, med, ray, R(rng.next_1d()));
+    if (!mei.valid) return mei;
+    R r = mei.combined != 0 ? mei.sigma_t[1] / mei.combined : R(0);
+    if (!(R(rng.next_1d()) >= r)) {
+      mei.t += run_t;
+      return mei;
+    }
+    ray.o = mei.p;
+    ray.maxt -= mei.t;
+    run_t += mei.t;
+  }
+}
+
+// estimate_transmittance (:450-491); with `ga`, the replay's gradient
+template <typename R>
+static R simple_ratio_tracking(const Scene<R> &S, int med, Ray<R> ray, R tmax, Pcg32 &rng, double adj,
+                               SimpleAdj<R> *ga) {
+  R T = 1;
+  ray.maxt = tmax;
+  while (true) {
+    MEI<R> mei = sample_interaction_twostates(S, med, ray, R(rng.next_1d()));
+    if (!mei.valid) break;
+    R tr = mei.combined != 0 ? mei.sigma_n[1] / mei.combined : mei.sigma_n[1];
+    if (ga && tr > 0 && S.s.med_sigma[med] == ga->param && adj != 0)
+      grid_scatter(S, ga->param, med, mei.p, -adj * double(S.s.med_scale[med]) / double(mei.sigma_n[1]), *ga->g);
+    T *= tr;
+    ray.o = mei.p;
+    ray.maxt -= mei.t;
+    if (T == 0) break;
+  }
+  return T;
+}
+
+// sample_emitter (:415-445)
+template <typename R>
+static V<R> simple_sample_emitter(const Scene<R> &S, int med, const V<R> &p, Pcg32 &rng,
+                                  typename Scene<R>::DS &ds, double adj, SimpleAdj<R> *ga) {
+  R u1 = R(rng.next_1d()), u2 = R(rng.next_1d());
+  V<R> ev = S.sample_emitter_direction(p, u1, u2, ds);
+  if (ds.pdf == 0) return V<R>();
+  Ray<R> r;
+  r.o = p;
+  r.d = ds.d;
+  r.maxt = std::numeric_limits<R>::infinity();
+  SI<R> si = S.intersect(r);
+  if (!si.valid) return V<R>();
+  return ev * simple_ratio_tracking(S, med, r, si.t, rng, adj, ga);
+}
+
+template <typename R>
+struct SimpleStart {
+  bool from_camera = true;
+  int depth = 0;
+  R last_pdf = 1;
+  SI<R> si;
+};
+
+template <typename R>
+static V<R> simple_sample(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &rng, Ray<R> ray,
+                          const SimpleStart<R> &start, SimpleAdj<R> *adj);
+
+template <typename R>
+static V<R> simple_recursive_li(const Scene<R> &S, const OracleIntegrator &I, const V<R> &p, int depth,
+                                Pcg32 &alt) {
+  using DS = typename Scene<R>::DS;
+  const R inv4pi = Consts<R>::inv_4pi;
+  V<R> result;
+  if (!(I.flags & 1)) {
+    DS ds;
+    V<R> em = simple_sample_emitter(S, 0, p, alt, ds, 0.0, (SimpleAdj<R> *)nullptr);
+    result = result + em * (inv4pi * mis_weight(ds.pdf, inv4pi));
+  }
+  (void)alt.next_1d();
+  R a = R(alt.next_1d()), b = R(alt.next_1d());
+  Ray<R> rec;
+  rec.o = p;
+  rec.d = uniform_sphere(a, b);
+  rec.maxt = std::numeric_limits<R>::infinity();
+  SimpleStart<R> st;
+  st.from_camera = false;
+  st.depth = depth + 1;
+  st.last_pdf = inv4pi;
+  st.si = S.intersect(rec);
+  return result + simple_sample(S, I, alt, rec, st, (SimpleAdj<R> *)nullptr);
+}
+
+// backpropagate_scattering_drt (:502-556); distance density proportional to
+// the majorant transmittance on the medium part of the segment, weight
+// T_ratio(s) / p(s)
+template <typename R>
+static void simple_drt(const Scene<R> &S, const OracleIntegrator &I, Ray<R> ray, const SI<R> &si, int depth,
+                       const V<R> &adjoint, SimpleAdj<R> &ga) {
+  const int med = 0;
+  Pcg32 &alt = *ga.alt;
+  ray.maxt = si.valid ? si.t : std::numeric_limits<R>::infinity();
+  R mint, maxt;
+  const bool hit = medium_interval(S, med, ray, mint, maxt);
+  const int sp = S.s.med_sigma[med];
+  const R scale = S.s.med_scale[med];
+  const R maj = std::max(S.maj_old[sp] * scale, S.maj_new[sp] * scale);
+  const R u = R(alt.next_1d());
+  const R len = maxt - mint;
+  if (!hit || !(len > 0) || !std::isfinite(len) || !(maj > 0) || !std::isfinite(maj)) return;
+  const R norm = -std::expm1(-maj * len);
+  const R s = -std::log1p(-u * norm) / maj;
+  const R pdf = maj * std::exp(-maj * s) / norm;
+  Ray<R> seg;
+  seg.o = ray.o + ray.d * mint;
+  seg.d = ray.d;
+  seg.maxt = std::numeric_limits<R>::infinity();
+  const R T = simple_ratio_tracking(S, med, seg, s, alt, 0.0, (SimpleAdj<R> *)nullptr);
+  const R weight = pdf > 0 ? T / pdf : R(0);
+  const V<R> p = ray.o + ray.d * (mint + s);
+  const R st = scale * S.grid(sp, 1, med, p).x;
+  const V<R> alb = albedo_at(S, med, p);
+  const V<R> li = simple_recursive_li(S, I, p, depth, alt);
+  const R mis = (I.flags & 8) ? R(1) : R(1) / (R(1) + st * st);
+  const V<R> a = adjoint * li * (mis * weight);
+  if (sp == ga.param && weight != 0)
+    grid_scatter(S, ga.param, med, p, double(dot(a, alb)) * double(scale), *ga.g);
+}
+
+// sample() of plugins/volpathsimple.py:51-296, primal (adj == nullptr) or
+// adjoint (adj->Lrem = primal radiance on entry).
+template <typename R>
+static V<R> simple_sample(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &rng, Ray<R> ray,
+                          const SimpleStart<R> &start, SimpleAdj<R> *adj) {
+  using DS = typename Scene<R>::DS;
+  const R inf = std::numeric_limits<R>::infinity();
+  const R inv4pi = Consts<R>::inv_4pi;
+  const int med = 0;
+  const bool use_nee = !(I.flags & 1), use_drt = !(I.flags & 2), drt_sub = !(I.flags & 4), drt_mis = !(I.flags & 8);
+  const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
+  const int sp = S.s.med_sigma[med];
+  const R scale = S.s.med_scale[med];
+  V<R> result, thr(1);
+  int depth = start.depth;
+  bool active = true, escaped = false, has_scattered = !start.from_camera;
+  R last_pdf = start.last_pdf;
+  SI<R> si = start.si;
+  if (start.from_camera) {
+    (void)rng.next_1d();  // colour channel
+    SI<R> s0 = S.intersect(ray);
+    escaped = !s0.valid;
+    active = s0.valid;
+    si = s0;
+    if (active) {
+      ray = spawn_ray(s0, ray.d);
+      si = S.intersect(ray);
+      active = si.valid;
+      ray.maxt = si.valid ? si.t : inf;
+    }
+  } else {
+    active = depth < max_depth;
+    ray.maxt = si.valid ? si.t : inf;
+  }
+  V<R> last_p = ray.o;
+  (void)rng.next_1d();  // seed of the gradient-only stream
+  V<R> wsum, cur;
+  bool res_valid = false;
+  int res_depth = 0;
+  Ray<R> res_ray;
+  SI<R> res_si;
+  while (active) {
+    R q = std::min(maxc(thr), R(0.99));
+    bool perform_rr = depth > I.rr_depth;
+    R urr = R(rng.next_1d());
+    if (!any_nz(thr) || (perform_rr && !(urr < q))) break;
+    if (perform_rr) thr = thr * (R(1) / q);
+    MEI<R> mei = simple_real_collision(S, med, ray, rng);
+    const bool did_scatter = mei.valid, did_escape = !mei.valid;
+    has_scattered = has_scattered || did_scatter;
+    const V<R> albedo = did_scatter ? albedo_at(S, med, mei.p) : V<R>(1);
+    if (adj) {
+      Pcg32 &alt = *adj->alt;
+      if (use_drt && drt_sub) {
+        wsum = wsum + thr;
+        R u = R(alt.next_1d());
+        V<R> ratio(thr.x / wsum.x, thr.y / wsum.y, thr.z / wsum.z);
+        if (u <= (ratio.x + ratio.y + ratio.z) * (R(1) / R(3))) {
+          cur = thr;
+          res_valid = true;
+          res_depth = depth;
+          res_ray = ray;
+          res_si = si;
+        }
+      }
+      if (use_drt && !drt_sub) simple_drt(S, I, ray, si, depth, adj->dL * thr, *adj);
+      if (did_scatter && (!use_drt || drt_mis) && mei.sigma_t[1] > 0) {
+        R s2 = mei.sigma_t[1] * mei.sigma_t[1];
+        R mis = use_drt ? s2 / (R(1) + s2) : R(1);
+        V<R> li(adj->Lrem.x / std::max(R(1e-8), albedo.x), adj->Lrem.y / std::max(R(1e-8), albedo.y),
+                adj->Lrem.z / std::max(R(1e-8), albedo.z));
+        V<R> dl = adj->dL * li;
+        if (sp == adj->param)
+          grid_scatter(S, adj->param, med, mei.p, double(mis * dot(dl, albedo) * scale / mei.sigma_t[1]), *adj->g);
+      }
+      {
+        R interval = did_escape ? si.t : mei.t;
+        R lr = dot(adj->dL, adj->Lrem);
+        for (int j = 0; j < 4; ++j) {
+          R tj = R(alt.next_1d()) * interval;
+          if (sp == adj->param && lr != 0 && std::isfinite(interval))
+            grid_scatter(S, adj->param, med, ray.o + ray.d * tj, double(-lr * scale * interval * R(0.25)),
+                         *adj->g);
+        }
+      }
+    }
+    thr = thr * albedo;
+    if (did_scatter) depth += 1;
+    active = did_scatter && depth < max_depth;
+    if (use_nee && did_scatter && active) {
+      const Pcg32 nee_rng = rng;
+      DS ds;
+      V<R> em = simple_sample_emitter(S, med, mei.p, rng, ds, 0.0, (SimpleAdj<R> *)nullptr);
+      R mis = mis_weight(ds.pdf, inv4pi);
+      V<R> c = thr * em * (inv4pi * mis);
+      if (adj) {
+        adj->Lrem = adj->Lrem - c;
+        R cs = dot(adj->dL, c);
+        if (cs != 0) {
+          Pcg32 r2 = nee_rng;
+          DS ds2;
+          (void)simple_sample_emitter(S, med, mei.p, r2, ds2, double(cs), adj);
+        }
+      } else {
+        result = result + c;
+      }
+    }
+    if (did_scatter) {
+      (void)rng.next_1d();
+      R a = R(rng.next_1d()), b = R(rng.next_1d());
+      ray.o = mei.p;
+      ray.d = uniform_sphere(a, b);
+      last_p = mei.p;
+      last_pdf = inv4pi;
+    }
+    if (did_scatter || did_escape) {
+      ray.maxt = inf;
+      si = S.intersect(ray);
+      ray.maxt = si.valid ? si.t : inf;
+    }
+    if (did_scatter && !si.valid) active = false;
+    if (did_escape) {
+      if (si.valid) ray = spawn_ray(si, ray.d);
+      escaped = true;
+    }
+  }
+  if (adj) {
+    if (use_drt && drt_sub && res_valid) {
+      R dm = (cur.x + cur.y + cur.z) * (R(1) / R(3));
+      R wm = (wsum.x + wsum.y + wsum.z) * (R(1) / R(3));
+      V<R> sw = dm != 0 ? cur * (wm / dm) : V<R>();
+      simple_drt(S, I, res_ray, res_si, res_depth, sw * adj->dL, *adj);
+    }
+    return adj->Lrem;
+  }
+  if (escaped) {
+    ray.maxt = inf;
+    SI<R> se = S.intersect(ray);
+    int em = S.si_emitter(se);
+    if (em >= 0 && !(depth <= 0 && I.hide_emitters)) {
+      DS ds = S.ds_from_hit(se, last_p, ray.d);
+      R em_pdf = has_scattered ? S.pdf_emitter_direction(ds) : R(0);
+      R mis = use_nee ? mis_weight(last_pdf, em_pdf) : R(1);
+      result = result + thr * S.emitter_eval(em, se, ray.d) * mis;
+    }
+  }
+  return result;
+}
+
 template <typename R>
 static Ray<R> camera_ray(const OracleScene &s, int px, int py, Pcg32 &rng) {
   int view_w = s.film_w / s.n_views;
@@ -1195,6 +1510,23 @@ static void adjoint(const OracleScene &sc, const OracleIntegrator &I, uint32_t s
         Ray<R> ray = camera_ray<R>(sc, px, py, rng);
         const Pcg32 rng0 = rng;
         V<R> L[2];
+        if (I.type == 2) {
+          SimpleStart<R> st;
+          V<R> L0 = simple_sample(S, I, rng, ray, st, (SimpleAdj<R> *)nullptr);
+          SimpleAdj<R> a;
+          a.dL = dL;
+          a.Lrem = L0;
+          a.param = param;
+          a.g = &bufs[tid];
+          Pcg32 alt;
+          uint32_t as = seed, ab = 2u;
+          tea(as, ab, 4, kTeaKey);
+          alt.seed(tea64(as, lane), tea64(lane, as));
+          a.alt = &alt;
+          rng = rng0;
+          (void)simple_sample(S, I, rng, ray, st, &a);
+          continue;
+        }
         volpath_twostates(S, I, rng, ray, L);
         if (!any_nz(L[1])) continue;
         Adj<R> a;
@@ -1239,7 +1571,10 @@ static void render(const OracleScene &sc, const OracleIntegrator &I, uint32_t se
         rng.seed(tea64(seed, lane), tea64(lane, seed));
         Ray<R> ray = camera_ray<R>(sc, px, py, rng);
         V<R> L[2];
-        if (I.type == 1)
+        if (I.type == 2) {
+          SimpleStart<R> st;
+          L[0] = L[1] = simple_sample(S, I, rng, ray, st, (SimpleAdj<R> *)nullptr);
+        } else if (I.type == 1)
           volpath_twostates(S, I, rng, ray, L);
         else
           path_twostates(S, I, rng, ray, L);
@@ -1288,7 +1623,7 @@ int oracle_render(const OracleScene *s, const OracleIntegrator *I, uint32_t seed
 // Volume adjoint (NEW state == param_new) into grad_out (grid-sized doubles).
 int oracle_adjoint(const OracleScene *s, const OracleIntegrator *I, uint32_t seed, int spp, int precision,
                    const double *grad_img, int param, double *grad_out, int n_threads) {
-  if (!s || !I || spp <= 0 || !grad_img || !grad_out || param < 0 || param >= s->n_params || I->type != 1) return -1;
+  if (!s || !I || spp <= 0 || !grad_img || !grad_out || param < 0 || param >= s->n_params || I->type < 1) return -1;
   if (precision == 64)
     oracle::adjoint<double>(*s, *I, seed, spp, grad_img, param, grad_out, n_threads);
   else

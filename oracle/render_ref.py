@@ -38,7 +38,8 @@ class OracleScene(C.Structure):
 
 
 class OracleIntegrator(C.Structure):
-    _fields_ = [("type", C.c_int), ("max_depth", C.c_int), ("rr_depth", C.c_int), ("hide_emitters", C.c_int)]
+    _fields_ = [("type", C.c_int), ("max_depth", C.c_int), ("rr_depth", C.c_int), ("hide_emitters", C.c_int),
+                ("flags", C.c_int)]
 
 
 def build():
@@ -144,14 +145,16 @@ def make_scene(desc, params_old, params_new, sensor=0):
 
 def render(desc, integrator_type, max_depth, rr_depth, seed, spp, params_old=None, params_new=None,
            precision=64, sensor=0, n_threads=None, with_variance=False, hide_emitters=False,
-           pixel_range=None):
-    """Returns (img_old, img_new[, var_old, var_new]) as (H, W, 3) float64."""
+           pixel_range=None, flags=0):
+    """Returns (img_old, img_new[, var_old, var_new]) as (H, W, 3) float64.
+    integrator_type: 0 path, 1 volpath (two states), 2 volpathsimple (NEW
+    state; both images are its render; flags switch off NEE / DRT parts)."""
     if params_new is None:
         params_new = [p.init for p in desc.params]
     if params_old is None:
         params_old = params_new
     s, keep = make_scene(desc, params_old, params_new, sensor)
-    I = OracleIntegrator(int(integrator_type), int(max_depth), int(rr_depth), int(hide_emitters))
+    I = OracleIntegrator(int(integrator_type), int(max_depth), int(rr_depth), int(hide_emitters), int(flags))
     sen = desc.sensors[sensor]
     shape = (sen.film_height, sen.film_width, 3)
     io = np.zeros(shape, np.float64)
@@ -172,12 +175,14 @@ def render(desc, integrator_type, max_depth, rr_depth, seed, spp, params_old=Non
 
 
 def adjoint(desc, max_depth, rr_depth, seed, spp, grad_img, param=0, params=None, precision=64,
-            n_threads=None, hide_emitters=False):
-    """Volume path-replay adjoint of the NEW state: d<grad_img, img>/d param."""
+            n_threads=None, hide_emitters=False, integrator_type=1, flags=0):
+    """Volume adjoint of the NEW state, d<grad_img, img>/d param: path replay
+    of the volume integrator (integrator_type 1) or volpathsimple's
+    differential-ratio-tracking adjoint (2)."""
     if params is None:
         params = [p.init for p in desc.params]
     s, keep = make_scene(desc, params, params, 0)
-    I = OracleIntegrator(1, int(max_depth), int(rr_depth), int(hide_emitters))
+    I = OracleIntegrator(int(integrator_type), int(max_depth), int(rr_depth), int(hide_emitters), int(flags))
     gi = np.ascontiguousarray(grad_img, np.float64)
     out = np.zeros(desc.params[param].shape[:3], np.float64)
     nt = n_threads or min(16, os.cpu_count() or 1)

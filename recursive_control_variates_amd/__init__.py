@@ -7,6 +7,7 @@ integrator ``cv``       CVIntegrator (plugins/cv_integrator.py)
 integrator ``meta``     MetaIntegrator (plugins/metaintegrator.py)
 ``twostateprb``         TwoStatePRBIntegrator (plugins/twostatepath.py)
 ``twostateprbvolpath``  TwoStatePRBVolpathIntegrator (plugins/twostatevolpath.py)
+``volpathsimple``       VolpathSimpleIntegrator (plugins/volpathsimple.py)
 ``prb``/``prbvolpath``  plain single-state renders (upstream base classes)
 bsdf ``twostate``       handled by the scene loader (plugins/twostatebsdf.py)
 medium ``twostatemedium`` handled by the scene loader (plugins/twostatemedium.py)
@@ -28,13 +29,14 @@ def register_integrator(name, ctor):
 def _register_defaults():
     from .cv_integrator import CVIntegrator, MetaIntegrator
     from .integrators import (PRBIntegrator, PRBVolpathIntegrator, TwoStatePRBIntegrator,
-                              TwoStatePRBVolpathIntegrator)
+                              TwoStatePRBVolpathIntegrator, VolpathSimpleIntegrator)
     register_integrator("prb", PRBIntegrator)
     register_integrator("path", PRBIntegrator)
     register_integrator("prbvolpath", PRBVolpathIntegrator)
     register_integrator("volpath", PRBVolpathIntegrator)
     register_integrator("twostateprb", TwoStatePRBIntegrator)
     register_integrator("twostateprbvolpath", TwoStatePRBVolpathIntegrator)
+    register_integrator("volpathsimple", VolpathSimpleIntegrator)
     register_integrator("meta", MetaIntegrator)
     register_integrator("cv", CVIntegrator)
 

@@ -30,6 +30,8 @@ EMITTER_AREA = 0
 EMITTER_CONSTANT = 1
 INTEGRATOR_PATH = 0
 INTEGRATOR_VOLPATH = 1
+INTEGRATOR_VOLPATH_SIMPLE = 2
+SIMPLE_NO_NEE, SIMPLE_NO_DRT, SIMPLE_NO_DRT_SUBSAMPLING, SIMPLE_NO_DRT_MIS = 1, 2, 4, 8
 LOSS_L1 = 0
 LOSS_L2 = 1
 
@@ -52,7 +54,7 @@ class SceneCheckError(RcvError):
 class IntegratorDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("max_depth", C.c_int32), ("rr_depth", C.c_int32),
                 ("hide_emitters", C.c_int32), ("samples_per_thread", C.c_int32),
-                ("block_size", C.c_int32)]
+                ("block_size", C.c_int32), ("flags", C.c_int32)]
 
 
 class MediumDesc(C.Structure):

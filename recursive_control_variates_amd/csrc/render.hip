@@ -18,6 +18,7 @@
 #include "path.hpp"
 #include "render_internal.hpp"
 #include "volpath.hpp"
+#include "volsimple.hpp"
 
 namespace rcv {
 namespace {
@@ -65,6 +66,9 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) 
       V3 L[NS];
       if (INTEG == RCV_INTEGRATOR_VOLPATH)
         volpath_sample<NS, false, F>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+      else if (INTEG == RCV_INTEGRATOR_VOLPATH_SIMPLE)
+        L[0] = simple_sample<false, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, nullptr, nullptr,
+                                       nullptr);
       else
         path_sample<NS, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
 #pragma unroll
@@ -124,6 +128,17 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) 
         const Ray ray = camera_ray(sen, px, py, rng);
         const Pcg32 rng0 = rng;
         V3 L[1];
+        if (INTEG == RCV_INTEGRATOR_VOLPATH_SIMPLE) {
+          // primal, then the adjoint with the same stream plus the
+          // gradient-only stream (DRT contributes even where L = 0)
+          L[0] = simple_sample<false, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, nullptr, nullptr,
+                                         nullptr);
+          AdjState adj{dL, L[0]};
+          rng = rng0;
+          Pcg32 alt = lane_rng(simple_alt_seed(seed), uint32_t(pixel * spp + s));
+          (void)simple_sample<true, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, &adj, &alt, &G);
+          continue;
+        }
         if (INTEG == RCV_INTEGRATOR_VOLPATH)
           volpath_sample<1, false, F>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
         else
@@ -211,11 +226,44 @@ static void launch_render_kernel(const RenderLaunch &L) {
                      L.seed, L.spp, L.spt, L.cl2, L.img_old, L.img_new);
 }
 
+// get_single_medium and the class docstring of plugins/volpathsimple.py
+// (:9-30): one shape with an interior medium and a null BSDF, the
+// environment as the only emitter.
+static int check_simple(const Scene *s, const char *fn) {
+  if (s->shapes.size() != 1)
+    return fail(RCV_E_SCENE, "%s: volpathsimple: Not supported: more than 1 shape in the scene (found %d).", fn,
+                int(s->shapes.size()));
+  if (s->media.size() != 1 || s->shapes[0].interior != 0)
+    return fail(RCV_E_SCENE, "%s: volpathsimple: Expected a single shape with an interior medium.", fn);
+  if (s->features() & kFeatSurfaces)
+    return fail(RCV_E_SCENE, "%s: volpathsimple: the medium boundary must use a null BSDF", fn);
+  if (s->emitters.size() != 1 || s->emitters[0].type != RCV_EMITTER_CONSTANT)
+    return fail(RCV_E_SCENE, "%s: volpathsimple: the only emitter must be an infinite light", fn);
+  return RCV_OK;
+}
+
+// with_features over the masks without surfaces (volpathsimple scenes).
+template <typename Fn>
+static void with_volume_features(int f, Fn &&fn) {
+  switch (f & (kFeatAlbedoGrid | kFeatSupergrid)) {
+    case 0: fn(std::integral_constant<int, 0>{}); break;
+    case kFeatAlbedoGrid: fn(std::integral_constant<int, kFeatAlbedoGrid>{}); break;
+    case kFeatSupergrid: fn(std::integral_constant<int, kFeatSupergrid>{}); break;
+    default: fn(std::integral_constant<int, kFeatAlbedoGrid | kFeatSupergrid>{}); break;
+  }
+}
+
 int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint32_t seed, int spp,
                   float *img_old, float *img_new, int ns, hipStream_t stream, const char *fn) {
   RCV_REQUIRE(s && s->committed, "%s: scene not committed", fn);
   RCV_REQUIRE(d, "%s: null integrator", fn);
-  RCV_REQUIRE(d->type == RCV_INTEGRATOR_PATH || d->type == RCV_INTEGRATOR_VOLPATH, "%s: bad integrator type", fn);
+  RCV_REQUIRE(d->type == RCV_INTEGRATOR_PATH || d->type == RCV_INTEGRATOR_VOLPATH ||
+                  d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE,
+              "%s: bad integrator type", fn);
+  if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE) {
+    RCV_REQUIRE(ns == 1, "%s: volpathsimple renders one state", fn);
+    if (int rc = check_simple(s, fn)) return rc;
+  }
   RCV_REQUIRE(sensor >= 0 && sensor < int(s->sensors.size()), "%s: bad sensor %d", fn, sensor);
   RCV_REQUIRE(spp > 0, "%s: spp must be positive", fn);
   RCV_REQUIRE(img_new && (ns == 1 || img_old), "%s: null image", fn);
@@ -225,13 +273,17 @@ int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint
   RCV_REQUIRE(n_pix * spp < (int64_t(1) << 32), "%s: pixels * spp must stay below 2^32", fn);
   int spt, cl2;
   if (int rc = plan_launch(d, n_pix, spp, spt, cl2)) return rc;
-  const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters};
+  const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters, d->flags};
   const int64_t threads = n_pix << cl2;
   const int block = d->block_size ? d->block_size : kDefaultBlock;
   const unsigned grid = ceil_div(threads, block);
   if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
   const RenderLaunch L{dim3(grid), block, stream, &s->view, I, sensor, seed, spp, spt, cl2, img_old, img_new};
-  if (d->type == RCV_INTEGRATOR_VOLPATH) {
+  if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE) {
+    with_volume_features(s->features(), [&](auto f) {
+      launch_render_kernel<1, RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>(L);
+    });
+  } else if (d->type == RCV_INTEGRATOR_VOLPATH) {
     if (ns == 2)
       with_features(s->features(), [&](auto f) { launch_render_kernel<2, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>(L); });
     else
@@ -254,6 +306,11 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   RCV_REQUIRE(sensor >= 0 && sensor < int(s->sensors.size()), "%s: bad sensor %d", fn, sensor);
   RCV_REQUIRE(spp > 0 && grad_img, "%s: bad spp or null grad_img", fn);
   RCV_REQUIRE(n_params >= 0 && (n_params == 0 || (param_ids && grad_ptrs)), "%s: bad gradient targets", fn);
+  RCV_REQUIRE(d->type == RCV_INTEGRATOR_PATH || d->type == RCV_INTEGRATOR_VOLPATH ||
+                  d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE,
+              "%s: bad integrator type", fn);
+  if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE)
+    if (int rc = check_simple(s, fn)) return rc;
   if (int rc = check_states(s, 1, fn)) return rc;
   GradTargets G;
   for (int i = 0; i < kMaxParams; ++i) G.g[i] = nullptr;
@@ -267,11 +324,17 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   RCV_REQUIRE(n_pix * spp < (int64_t(1) << 32), "%s: pixels * spp must stay below 2^32", fn);
   int spt, cl2;
   if (int rc = plan_launch(d, n_pix, spp, spt, cl2)) return rc;
-  const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters};
+  const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters, d->flags};
   const int block = d->block_size ? d->block_size : kDefaultBlock;
   const unsigned grid = ceil_div(n_pix << cl2, block);
   if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
-  if (d->type == RCV_INTEGRATOR_VOLPATH)
+  if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE)
+    with_volume_features(s->features(), [&](auto f) {
+      hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>), dim3(grid),
+                         dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img,
+                         G);
+    });
+  else if (d->type == RCV_INTEGRATOR_VOLPATH)
     with_features(s->features(), [&](auto f) {
       hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), dim3(grid), dim3(block),
                          stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);

@@ -25,6 +25,7 @@ struct IntegArgs {
   int max_depth;
   int rr_depth;
   int hide_emitters;
+  int flags;  // integrator-specific switches (volpathsimple: kSimpleNo*)
 };
 
 struct GradTargets {

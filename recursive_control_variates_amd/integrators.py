@@ -125,6 +125,35 @@ class PRBVolpathIntegrator(RenderIntegrator):
     INTEGRATOR_TYPE = N.INTEGRATOR_VOLPATH
 
 
+class VolpathSimpleIntegrator(RenderIntegrator):
+    """'volpathsimple' (plugins/volpathsimple.py:23-48): one medium in a
+    null-BSDF boundary lit by an environment; its adjoint uses differential
+    ratio tracking. Props ``use_nee``, ``use_drt``, ``use_drt_subsampling``,
+    ``use_drt_mis`` (all True by default), ``max_depth``, ``rr_depth``,
+    ``hide_emitters``. The reference uses it as the ``adjoint_integrator`` of
+    volume optimisations (experiments.py:131)."""
+
+    INTEGRATOR_TYPE = N.INTEGRATOR_VOLPATH_SIMPLE
+
+    def __init__(self, props=None):
+        super().__init__(props)
+        p = _props(props)
+        self.use_nee = bool(p.get("use_nee", True))
+        self.use_drt = bool(p.get("use_drt", True))
+        self.use_drt_subsampling = bool(p.get("use_drt_subsampling", True))
+        self.use_drt_mis = bool(p.get("use_drt_mis", True))
+
+    def desc(self) -> N.IntegratorDesc:
+        d = super().desc()
+        d.flags = ((0 if self.use_nee else N.SIMPLE_NO_NEE) | (0 if self.use_drt else N.SIMPLE_NO_DRT)
+                   | (0 if self.use_drt_subsampling else N.SIMPLE_NO_DRT_SUBSAMPLING)
+                   | (0 if self.use_drt_mis else N.SIMPLE_NO_DRT_MIS))
+        return d
+
+    def __repr__(self):
+        return f"VolpathSimpleIntegrator[max_depth = {self.max_depth}, use_drt = {self.use_drt}]"
+
+
 class _TwoStateMixin:
     def _check_scene(self, scene):
         if getattr(self, "_checked_scene", None) is scene:

@@ -77,7 +77,10 @@ def test_batch_of_one_follows_driver(cuda):
     b.step(1)
     np.testing.assert_allclose(b.image(0).cpu().numpy(), ref_imgs[0].cpu().numpy(), rtol=1e-5, atol=1e-6)
     b.step(2)
-    np.testing.assert_allclose(b.param(0).cpu().numpy(), ref_param.cpu().numpy(), rtol=1e-3, atol=1e-4)
+    # gradients are float-atomic sums in a different order (the batch splits
+    # a pixel's samples over more threads); Adam's first steps move by about
+    # lr * sign(g), so near-zero gradients may differ by up to 2 lr per step
+    _params_close(b.param(0), ref_param, 0.02, 3)
     it, _ = b.job_state(0)
     assert it == 3
 
