@@ -46,9 +46,10 @@ def parse():
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--mode", choices=("step", "batch", "serve"), default="step",
+    ap.add_argument("--mode", choices=("step", "batch", "serve", "diff"), default="step",
                     help="step: the volume optimisation step (default line); batch: 64 concurrent jobs in one "
-                         "graph-replayed iteration; serve: continuous batching under Poisson step requests")
+                         "graph-replayed iteration; serve: continuous batching under Poisson step requests; "
+                         "diff: adjoints at 128^3 density + albedo, 4 x 512^2, with finite-difference checks")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--profile-target", action="store_true",
@@ -418,6 +419,79 @@ def bench_serve(args):
             "at_70pct_capacity": below}
 
 
+def bench_diff(args):
+    """Config-3 style: 128^3 density + 128^3 x 3 albedo grid, 4 views of
+    512^2, 32 spp, max_depth 16. Times the two-state render and both adjoints
+    (path replay; volpathsimple's DRT adjoint) for d/d(density) and
+    d/d(albedo) at once, and checks each adjoint against central finite
+    differences of two-state renders (shared paths) along random directions
+    supported on 32 seed-chosen voxels of each grid."""
+    import numpy as np
+    import torch
+    import recursive_control_variates_amd as rcv
+    from recursive_control_variates_amd import scenes
+    res, grid, spp = 512, 128, 32
+    dens = scenes.blob_grid(grid, 1)
+    alb = scenes.albedo_grid(grid, 1)
+    sc = rcv.load_dict(scenes.volume_scene_dict(dens, film_res=res, spp=spp, albedo_grid_data=alb))
+    p = rcv.traverse(sc)
+    dk, ak = scenes.VOLUME_KEY, scenes.ALBEDO_KEY
+    ts = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 16, "rr_depth": 16})
+    prb = rcv.load_dict({"type": "prbvolpath", "max_depth": 16, "rr_depth": 16})
+    simple = rcv.load_dict({"type": "volpathsimple", "max_depth": 16, "rr_depth": 16})
+    n_pix = res * res * 4
+    rng = np.random.default_rng(7)
+    gimg = torch.from_numpy(rng.uniform(0.5, 1.5, (res, 4 * res, 3)).astype(np.float32) / (n_pix * 3)).cuda()
+    xd = torch.from_numpy(dens).cuda().requires_grad_(True)
+    xa = torch.from_numpy(alb).cuda().requires_grad_(True)
+    p[dk] = xd
+    p[ak] = xa
+    p.update()
+    p.update()
+    out = {"metric": "config 3: two-state render and adjoints at 128^3 density + albedo grid, 4 x 512^2, 32 spp",
+           "samples_per_launch": n_pix * spp}
+    ts.render_twostates(sc, 0, 1, spp)
+    torch.cuda.synchronize()
+    out["twostate_ms"] = round(_event_time(lambda r: ts.render_twostates(sc, 0, 10 + r, spp), 3), 3)
+    for name, integ in (("prb", prb), ("volpathsimple", simple)):
+        integ.adjoint(sc, p, gimg, 0, 2, spp)
+        torch.cuda.synchronize()
+        out[f"adjoint_{name}_ms"] = round(_event_time(lambda r: integ.adjoint(sc, p, gimg, 0, 20 + r, spp), 3), 3)
+    log(f"[bench] diff timings {out}")
+    # finite differences along directions on 32 seed-chosen voxels
+    fd = {}
+    for key, base in ((dk, dens), (ak, alb)):
+        flat = base.reshape(-1, base.shape[-1])
+        strong = np.flatnonzero(dens.reshape(-1) > 0.3 * dens.max())
+        vox = np.random.default_rng(11).choice(strong, 32, replace=False)
+        rows = []
+        for d_i in range(2):
+            delta = np.zeros_like(flat)
+            delta[vox] = np.random.default_rng(100 + d_i).uniform(-1, 1, (32, flat.shape[1]))
+            delta = delta.reshape(base.shape).astype(np.float32)
+            eps = 1e-2 * float(np.abs(base).max())
+            f_vals, a_vals = [], {"prb": [], "volpathsimple": []}
+            for s in range(4):
+                p[key] = torch.from_numpy(base + eps * delta).cuda()
+                p.update()
+                p[key] = torch.from_numpy(base - eps * delta).cuda()
+                p.update()
+                H, F = ts.render_twostates(sc, 0, 500 + s, 64)
+                f_vals.append(float(((H - F) * gimg).sum()) / (2 * eps))
+                p[key] = torch.from_numpy(base).cuda().requires_grad_(True)
+                p.update()
+                p.update()
+                for name, integ in (("prb", prb), ("volpathsimple", simple)):
+                    g = integ.adjoint(sc, p, gimg, 0, 700 + s, 64)[key]
+                    a_vals[name].append(float((g * torch.from_numpy(delta).cuda()).sum()))
+            # [mean, standard error] over 4 seeds
+            rows.append({"fd": [float(np.mean(f_vals)), float(np.std(f_vals, ddof=1) / 2)],
+                         **{n: [float(np.mean(v)), float(np.std(v, ddof=1) / 2)] for n, v in a_vals.items()}})
+        fd[key] = rows
+    out["finite_differences"] = fd
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -437,6 +511,9 @@ def main():
         return
     if args.mode == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
+        return
+    if args.mode == "diff":
+        print(json.dumps(bench_diff(args)), flush=True)
         return
     st = Stepper(args, "cv_ps")
     log("[bench] reference image rendered")
