@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--max-depth", type=int, default=16)
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--adjoint", choices=("prb", "volpathsimple"), default="prb",
+                    help="adjoint integrator of the step: path replay of the primal integrator, or "
+                         "volpathsimple's differential-ratio-tracking adjoint (the reference's choice for its "
+                         "volume experiments, experiments.py:131)")
     ap.add_argument("--mode", choices=("step", "batch", "serve", "diff"), default="step",
                     help="step: the volume optimisation step (default line); batch: 64 concurrent jobs in one "
                          "graph-replayed iteration; serve: continuous batching under Poisson step requests; "
@@ -67,7 +71,7 @@ def profile_target(args):
     st.base.render_twostates(st.sc, 0, 5000, args.spp)
     st.params.update(st.opt)
     g = torch.full(st.sc.film_shape(0), 1e-6, device="cuda")
-    st.base.adjoint(st.sc, st.params, g, 0, 6000, args.spp)
+    st.adj.adjoint(st.sc, st.params, g, 0, 6000, args.spp)
     x = torch.ones(64 * 1024 * 1024, device="cuda")
     y = torch.empty_like(x)
     y.copy_(x)
@@ -109,7 +113,10 @@ class Stepper:
         self.sc, self.base, self.target, init = build_problem(args, method)
         self.key = scenes.VOLUME_KEY
         self.img_ref = img_ref if img_ref is not None else reference_image(args, self.sc, self.base, self.target)
-        self.meta = MetaIntegrator({"method": method, "integrator": self.base})
+        import recursive_control_variates_amd as rcv
+        self.adj = self.base if getattr(args, "adjoint", "prb") == "prb" else rcv.load_dict(
+            {"type": "volpathsimple", "max_depth": args.max_depth, "rr_depth": args.max_depth})
+        self.meta = MetaIntegrator({"method": method, "integrator": self.base, "adjoint_integrator": self.adj})
         self.params = traverse(self.sc)
         self.opt = Adam(lr=0.01)
         self.opt.set_clamp(self.key, 0.0, 1e9)
@@ -186,7 +193,7 @@ def kernel_roofline(args, st):
     separate launch so the counting atomics stay out of the timing. The
     adjoint is the dominant kernel of a step."""
     import torch
-    sc, integ = st.sc, st.base
+    sc, integ, adj = st.sc, st.base, st.adj
     n_pix = args.res * args.res * 4
     # two-state render: 2 states x 8 trilinear corners x 4 B per tentative
     # collision and per ratio-tracking step; 2 images x 12 B per pixel
@@ -202,16 +209,18 @@ def kernel_roofline(args, st):
     gimg = torch.full(sc.film_shape(0), 1.0 / (n_pix * 3), device="cuda")
     st.params.update(st.opt)
     sc.set_counting(False)
-    integ.adjoint(sc, st.params, gimg, 0, 77, args.spp)
+    adj.adjoint(sc, st.params, gimg, 0, 77, args.spp)
     torch.cuda.synchronize()
-    a_ms = _event_time(lambda r: integ.adjoint(sc, st.params, gimg, 0, 2000 + r, args.spp), args.kernel_reps)
+    a_ms = _event_time(lambda r: adj.adjoint(sc, st.params, gimg, 0, 2000 + r, args.spp), args.kernel_reps)
     sc.set_counting(True)
-    integ.adjoint(sc, st.params, gimg, 0, 2000, args.spp)
+    adj.adjoint(sc, st.params, gimg, 0, 2000, args.spp)
     ac = sc.read_counters()
     sc.set_counting(False)
     a_bytes = 32 * (ac["collisions"] + ac["nee_steps"]) + 4 * ac["grad_atomics"] + 12 * n_pix
     out = {}
-    for name, ms, by, cnt, kern in (("volume_adjoint", a_ms, a_bytes, ac, "adjoint_kernel<VOLPATH> (path replay)"),
+    akern = ("adjoint_kernel<VOLPATH> (path replay)" if adj is integ
+             else "adjoint_kernel<VOLPATH_SIMPLE> (differential ratio tracking)")
+    for name, ms, by, cnt, kern in (("volume_adjoint", a_ms, a_bytes, ac, akern),
                                     ("volume_twostate", r_ms, r_bytes, rc,
                                      "render_kernel<2, VOLPATH> (two-state delta tracking)")):
         ach = by / (ms * 1e-3) / 1e9
@@ -228,7 +237,7 @@ def supergrid_compare(args, st, factor=8):
     tentative-collision counts from the device counters. Same samples, so the
     difference is the tracker's work."""
     import torch
-    sc, integ = st.sc, st.base
+    sc, integ, adj = st.sc, st.base, st.adj
     n_pix = args.res * args.res * 4
     gimg = torch.full(sc.film_shape(0), 1.0 / (n_pix * 3), device="cuda")
     st.params.update(st.opt)
@@ -238,11 +247,11 @@ def supergrid_compare(args, st, factor=8):
         integ.render_twostates(sc, 0, 999, args.spp)
         torch.cuda.synchronize()
         r_ms = _event_time(lambda r: integ.render_twostates(sc, 0, 3000 + r, args.spp), args.kernel_reps)
-        a_ms = _event_time(lambda r: integ.adjoint(sc, st.params, gimg, 0, 4000 + r, args.spp), args.kernel_reps)
+        a_ms = _event_time(lambda r: adj.adjoint(sc, st.params, gimg, 0, 4000 + r, args.spp), args.kernel_reps)
         sc.set_counting(True)
         integ.render_twostates(sc, 0, 3000, args.spp)
         rc = sc.read_counters()
-        integ.adjoint(sc, st.params, gimg, 0, 4000, args.spp)
+        adj.adjoint(sc, st.params, gimg, 0, 4000, args.spp)
         ac = sc.read_counters()
         sc.set_counting(False)
         out[f"factor_{f}"] = {"twostate_ms": round(r_ms, 3), "adjoint_ms": round(a_ms, 3),
@@ -471,7 +480,7 @@ def bench_diff(args):
             delta = delta.reshape(base.shape).astype(np.float32)
             eps = 1e-2 * float(np.abs(base).max())
             f_vals, a_vals = [], {"prb": [], "volpathsimple": []}
-            for s in range(4):
+            for s in range(8):
                 p[key] = torch.from_numpy(base + eps * delta).cuda()
                 p.update()
                 p[key] = torch.from_numpy(base - eps * delta).cuda()
@@ -484,9 +493,10 @@ def bench_diff(args):
                 for name, integ in (("prb", prb), ("volpathsimple", simple)):
                     g = integ.adjoint(sc, p, gimg, 0, 700 + s, 64)[key]
                     a_vals[name].append(float((g * torch.from_numpy(delta).cuda()).sum()))
-            # [mean, standard error] over 4 seeds
-            rows.append({"fd": [float(np.mean(f_vals)), float(np.std(f_vals, ddof=1) / 2)],
-                         **{n: [float(np.mean(v)), float(np.std(v, ddof=1) / 2)] for n, v in a_vals.items()}})
+            # [mean, standard error] over 8 seeds
+            se = lambda v: float(np.std(v, ddof=1) / np.sqrt(len(v)))
+            rows.append({"fd": [float(np.mean(f_vals)), se(f_vals)],
+                         **{n: [float(np.mean(v)), se(v)] for n, v in a_vals.items()}})
         fd[key] = rows
     out["finite_differences"] = fd
     return out
@@ -539,7 +549,8 @@ def main():
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (procedural blob-grid target, seed 1)",
         "config": {"workload": f"heterogeneous volume {args.grid}^3, 4 views {args.res}x{args.res}, "
-                               f"spp {args.spp}/{args.spp}, max_depth {args.max_depth}, cv_ps, Adam lr 0.01, L1",
+                               f"spp {args.spp}/{args.spp}, max_depth {args.max_depth}, cv_ps, Adam lr 0.01, L1, "
+                               f"adjoint {args.adjoint}",
                    "global_batch": n_pix * args.spp, "parallelism": "single GPU"},
         "p50_step_ms": round(float(np.percentile(per, 50)), 4),
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
