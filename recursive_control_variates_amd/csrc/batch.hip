@@ -178,14 +178,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
       if (pixel < J.n_pix) {
         const int px = int(pixel % J.film_w), py = int(pixel / J.film_w);
         const int s_end = sample_end(chunk, J.spp, J.cl2);
-        for (int s = sample_end(chunk - 1, J.spp, J.cl2); s < s_end; ++s) {
+        if (INTEG == RCV_INTEGRATOR_VOLPATH)
+          volpath_render_samples<NS, F>(S, J.integ, sen, seed, px, py, pixel, J.spp,
+                                        sample_end(chunk - 1, J.spp, J.cl2), s_end, acc, stack, cnt);
+        else for (int s = sample_end(chunk - 1, J.spp, J.cl2); s < s_end; ++s) {
           Pcg32 rng = lane_rng(seed, uint32_t(pixel * J.spp + s));
           const Ray ray = camera_ray(sen, px, py, rng);
           V3 L[NS];
-          if (INTEG == RCV_INTEGRATOR_VOLPATH)
-            volpath_sample<NS, false, F>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
-          else
-            path_sample<NS, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
+          path_sample<NS, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
 #pragma unroll
           for (int q = 0; q < NS; ++q) acc[q] += L[q];
         }
@@ -325,25 +325,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void b
         const V3 dL = v3(J.grad_img[3 * pixel] / inv, J.grad_img[3 * pixel + 1] / inv,
                          J.grad_img[3 * pixel + 2] / inv);
         const uint32_t seed = iter_seed_grad(J);
-        if (any_nonzero(dL)) {
+        if (any_nonzero(dL) && INTEG == RCV_INTEGRATOR_VOLPATH) {
+          volpath_adjoint_samples<F>(S, J.integ, sen, seed, px, py, pixel, J.spp_g,
+                                     sample_end(chunk - 1, J.spp_g, J.cl2_g), sample_end(chunk, J.spp_g, J.cl2_g), dL,
+                                     &J.gt, stack, cnt);
+        } else if (any_nonzero(dL)) {
           const int s_end = sample_end(chunk, J.spp_g, J.cl2_g);
           for (int s = sample_end(chunk - 1, J.spp_g, J.cl2_g); s < s_end; ++s) {
             Pcg32 rng = lane_rng(seed, uint32_t(pixel * J.spp_g + s));
             const Ray ray = camera_ray(sen, px, py, rng);
             const Pcg32 rng0 = rng;
             V3 L[1];
-            if (INTEG == RCV_INTEGRATOR_VOLPATH)
-              volpath_sample<1, false, F>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
-            else
-              path_sample<1, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
+            path_sample<1, false>(S, J.integ, rng, ray, L, stack, cnt, nullptr, nullptr);
             if (!any_nonzero(L[0])) continue;
             AdjState adj{dL, L[0]};
             rng = rng0;
             V3 L2[1];
-            if (INTEG == RCV_INTEGRATOR_VOLPATH)
-              volpath_sample<1, true, F>(S, J.integ, rng, ray, L2, stack, cnt, &adj, &J.gt);
-            else
-              path_sample<1, true>(S, J.integ, rng, ray, L2, stack, cnt, &adj, &J.gt);
+            path_sample<1, true>(S, J.integ, rng, ray, L2, stack, cnt, &adj, &J.gt);
           }
         }
       }

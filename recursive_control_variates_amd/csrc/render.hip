@@ -60,13 +60,14 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) 
   if (pixel < n_pix) {
     const int px = int(pixel % film_w), py = int(pixel / film_w);
     const int s_end = sample_end(chunk, spp, chunk_log2);
-    for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s) {
+    if (INTEG == RCV_INTEGRATOR_VOLPATH)
+      volpath_render_samples<NS, F>(S, I, sen, seed, px, py, pixel, spp, sample_end(chunk - 1, spp, chunk_log2),
+                                    s_end, acc, stack, cnt);
+    else for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s) {
       Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
       const Ray ray = camera_ray(sen, px, py, rng);
       V3 L[NS];
-      if (INTEG == RCV_INTEGRATOR_VOLPATH)
-        volpath_sample<NS, false, F>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
-      else if (INTEG == RCV_INTEGRATOR_VOLPATH_SIMPLE)
+      if (INTEG == RCV_INTEGRATOR_VOLPATH_SIMPLE)
         L[0] = simple_sample<false, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, nullptr, nullptr,
                                        nullptr);
       else
@@ -121,7 +122,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     const int px = int(pixel % film_w), py = int(pixel / film_w);
     const float inv = float(spp);
     const V3 dL = v3(grad_img[3 * pixel] / inv, grad_img[3 * pixel + 1] / inv, grad_img[3 * pixel + 2] / inv);
-    if (any_nonzero(dL)) {
+    if (any_nonzero(dL) && INTEG == RCV_INTEGRATOR_VOLPATH) {
+      volpath_adjoint_samples<F>(S, I, sen, seed, px, py, pixel, spp, sample_end(chunk - 1, spp, chunk_log2),
+                                 sample_end(chunk, spp, chunk_log2), dL, &G, stack, cnt);
+    } else if (any_nonzero(dL)) {
       const int s_end = sample_end(chunk, spp, chunk_log2);
       for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s) {
         Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
@@ -139,18 +143,12 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) 
           (void)simple_sample<true, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, &adj, &alt, &G);
           continue;
         }
-        if (INTEG == RCV_INTEGRATOR_VOLPATH)
-          volpath_sample<1, false, F>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
-        else
-          path_sample<1, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+        path_sample<1, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
         if (!any_nonzero(L[0])) continue;
         AdjState adj{dL, L[0]};
         rng = rng0;
         V3 L2[1];
-        if (INTEG == RCV_INTEGRATOR_VOLPATH)
-          volpath_sample<1, true, F>(S, I, rng, ray, L2, stack, cnt, &adj, &G);
-        else
-          path_sample<1, true>(S, I, rng, ray, L2, stack, cnt, &adj, &G);
+        path_sample<1, true>(S, I, rng, ray, L2, stack, cnt, &adj, &G);
       }
     }
   }

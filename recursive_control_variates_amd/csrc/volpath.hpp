@@ -374,50 +374,68 @@ struct AdjState {
   V3 Lrem;  // radiance still to be gathered along the replayed path
 };
 
-// One camera sample. NS = 2: L[0] = old (H), L[1] = new (F). REPLAY (NS = 1):
-// second pass of path-replay backpropagation; the primal radiance has been
-// stored in adj->Lrem by the caller.
-template <int NS, bool REPLAY, int F>
-__device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, Ray ray, V3 L[NS],
-                               const LdsStack &stack, LocalCounters &cnt, AdjState *adj,
-                               const GradTargets *G) {
-  V3 thr[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    L[k] = v3(0.f);
-    thr[k] = v3(1.f);
-  }
-  float eta = 1.0f;
-  int depth = 0;
-  bool specular_chain = true;
+// One camera sample of the two-state volumetric path tracer, advanced one
+// bounce per call (the body of the reference's loop, :175-380), so that a
+// kernel can start a lane's next camera sample as soon as its path ends
+// instead of waiting for the longest path of its wavefront. NS = 2: L[0] =
+// old (H), L[1] = new (F). ADJ (NS = 1) enables the replay pass of path-replay
+// backpropagation (replay = true): with the primal radiance in adj.Lrem, the
+// same random numbers are replayed and d(log weight)/d(theta) at each
+// collision times the radiance still to come is scattered into the gradients.
+template <int NS, bool ADJ, int F>
+struct VolPath {
+  Pcg32 rng;
+  Ray ray;
+  V3 thr[NS], L[NS];
   SurfInt si;
-  si.valid = false;
-  si.t = kInf;
-  si.shape = -1;
-  si.p = v3(0.f);
-  si.n = v3(0.f, 0.f, 1.f);
-  bool needs_int = true;
-  V3 last_p = v3(0.f);
-  float last_pdf = 1.0f;
-  int medium = -1;
-  const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
-  (void)rng.next_1d();  // colour channel for free-flight sampling (:171-173)
-  cnt.c[kCntPaths]++;
+  V3 last_p;
+  float last_pdf;
+  int depth, medium;
+  bool needs_int, specular_chain, replay;
+  AdjState adj;
 
-  while (true) {
+  __device__ void begin(const Pcg32 &r, const Ray &camera, LocalCounters &cnt, bool is_replay) {
+    rng = r;
+    ray = camera;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      L[k] = v3(0.f);
+      thr[k] = v3(1.f);
+    }
+    depth = 0;
+    specular_chain = true;
+    si.valid = false;
+    si.t = kInf;
+    si.shape = -1;
+    si.p = v3(0.f);
+    si.n = v3(0.f, 0.f, 1.f);
+    si.u = si.v = 0.0f;
+    needs_int = true;
+    last_p = v3(0.f);
+    last_pdf = 1.0f;
+    medium = -1;
+    replay = ADJ && is_replay;
+    (void)rng.next_1d();  // colour channel for free-flight sampling (:171-173)
+    cnt.c[kCntPaths]++;
+  }
+
+  // One bounce; false once the path has ended.
+  __device__ bool bounce(const SceneView &S, const IntegArgs &I, const LdsStack &stack, LocalCounters &cnt,
+                         const GradTargets *G) {
+    const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
     // ---- Russian roulette (:182-188)
     bool nonzero = false;
-#pragma unroll
+  #pragma unroll
     for (int k = 0; k < NS; ++k) nonzero = nonzero || any_nonzero(thr[k]);
-    if (!nonzero) break;
+    if (!nonzero) return false;
     const V3 tm = NS == 2 ? vmax(thr[NS - 1], thr[0]) : thr[0];
-    const float q = fminf(hmax(tm) * (eta * eta), 0.99f);
+    const float q = fminf(hmax(tm), 0.99f);  // eta = 1: no dielectric interfaces
     const bool perform_rr = depth > I.rr_depth;
     const float urr = rng.next_1d();
-    if (perform_rr && !(urr < q)) break;
+    if (perform_rr && !(urr < q)) return false;
     if (perform_rr) {
       const float rq = 1.0f / q;
-#pragma unroll
+  #pragma unroll
       for (int k = 0; k < NS; ++k) thr[k] = thr[k] * rq;
     }
 
@@ -450,7 +468,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
     // ---- null / real decision with the averaged probability (:219-231)
     bool act_null = false, act_scatter = false;
     V3 w[NS];
-#pragma unroll
+  #pragma unroll
     for (int k = 0; k < NS; ++k) w[k] = v3(1.f);
     if (active_medium) {
       const float maj = mev.maj;
@@ -490,15 +508,15 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
       }
     }
     if (active_medium) {
-#pragma unroll
+  #pragma unroll
       for (int k = 0; k < NS; ++k) thr[k] *= w[k];
     }
-    if (REPLAY && (act_null || act_scatter)) {
+    if (ADJ && replay && (act_null || act_scatter)) {
       // d log(weight) / d theta at this collision times the radiance that
       // the weight multiplies (everything gathered after this point).
       const MediumInfo &M = S.media[medium];
       float *gd = G->g[M.sigma_param];
-      const float lr = dot(adj->dL, adj->Lrem);
+      const float lr = dot(adj.dL, adj.Lrem);
       if (gd && lr != 0.0f) {
         if (act_null) {
           if (mev.sn[0] != 0.0f) scatter_grid(gd, mev.foot, -lr * M.scale / mev.sn[0]);
@@ -508,7 +526,7 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
         cnt.c[kCntGradAtomics] += 8;
       }
       if ((F & kFeatAlbedoGrid) && act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
-        const V3 dl = adj->dL * adj->Lrem;
+        const V3 dl = adj.dL * adj.Lrem;
         const V3 ga = v3(mev.alb.x != 0.f ? dl.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? dl.y / mev.alb.y : 0.f,
                          mev.alb.z != 0.f ? dl.z / mev.alb.z : 0.f);
         GridFoot fa;
@@ -547,11 +565,11 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
         const float em_pdf = pdf_emitter_direction(S, ds);
         const V3 emitted = emitter_eval(S, em, si, ray.d);
         const float mis = mis_weight(last_pdf, em_pdf);
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < NS; ++k) {
           const V3 c = count_direct ? thr[k] * emitted : thr[k] * mis * emitted;
           L[k] += c;
-          if (REPLAY) adj->Lrem -= c;
+          if (ADJ && replay) adj.Lrem -= c;
         }
       }
     }
@@ -585,31 +603,33 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
       float nee_pdf;
       if (active_e_surface) {
         const V3 wo = fr.to_local(ds.d);
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < NS; ++k)
           nw[k] = diffuse_eval(diffuse_reflectance(S, bsdf, state_index<NS>(k), si.u, si.v), wi, wo);
         nee_pdf = diffuse_pdf(wi, wo);
       } else {
-#pragma unroll
+  #pragma unroll
         for (int k = 0; k < NS; ++k) nw[k] = v3(kInv4Pi);
         nee_pdf = kInv4Pi;
       }
       const float mis_em = mis_weight(ds.pdf, nee_pdf);
       V3 c[NS];
-#pragma unroll
+  #pragma unroll
       for (int k = 0; k < NS; ++k) {
         c[k] = thr[k] * nw[k] * mis_em * em[k];
         L[k] += c[k];
       }
-      if (REPLAY) {
-        const float cs = dot(adj->dL, c[0]);
+      if (ADJ && replay) {
+        const float cs = dot(adj.dL, c[0]);
         if (cs != 0.0f && any_nonzero(c[0])) {
           Pcg32 r2 = rng_nee;
           V3 em2[NS];
           DirSample ds2;
-          nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G);
+          if constexpr (ADJ)
+              nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs,
+                                         G);
         }
-        adj->Lrem -= c[0];
+        adj.Lrem -= c[0];
       }
     }
 
@@ -663,7 +683,61 @@ __device__ void volpath_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rn
       const ShapeInfo sh = S.shapes[si.shape];
       if (sh.interior != sh.exterior) medium = dot(si.n, ray.d) > 0.0f ? sh.exterior : sh.interior;
     }
-    if (!(alive && (active_surface || active_medium))) break;
+    return alive && (active_surface || active_medium);
+  }
+};
+
+// The camera samples [s0, s1) of one pixel; their radiance is added to acc in
+// sample order. A lane starts its next sample in the bounce after its path
+// ends.
+template <int NS, int F>
+__device__ void volpath_render_samples(const SceneView &S, const IntegArgs &I, const SensorInfo &sen, uint32_t seed,
+                                       int px, int py, int64_t pixel, int spp, int s0, int s1, V3 acc[NS],
+                                       const LdsStack &stack, LocalCounters &cnt) {
+  VolPath<NS, false, F> t;
+  bool live = false;
+  int s = s0;
+  while (true) {
+    if (!live) {
+      if (s >= s1) break;
+      Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+      const Ray r = camera_ray(sen, px, py, rng);
+      t.begin(rng, r, cnt, false);
+    }
+    live = t.bounce(S, I, stack, cnt, nullptr);
+    if (!live) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) acc[k] += t.L[k];
+      ++s;
+    }
+  }
+}
+
+// Path-replay adjoint of the camera samples [s0, s1) of one pixel (NEW
+// state): per sample a primal pass for its radiance L, then, when L is not
+// zero, the replay pass with the same random numbers. dimage/dL = 1/spp (box
+// filter) is folded into dL.
+template <int F>
+__device__ void volpath_adjoint_samples(const SceneView &S, const IntegArgs &I, const SensorInfo &sen, uint32_t seed,
+                                        int px, int py, int64_t pixel, int spp, int s0, int s1, V3 dL,
+                                        const GradTargets *G, const LdsStack &stack, LocalCounters &cnt) {
+  VolPath<1, true, F> t;
+  bool live = false, want_replay = false;
+  int s = s0;
+  while (true) {
+    if (!live) {
+      if (s >= s1) break;
+      const AdjState a{dL, t.L[0]};
+      Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+      const Ray r = camera_ray(sen, px, py, rng);
+      t.begin(rng, r, cnt, want_replay);
+      if (want_replay) t.adj = a;
+    }
+    live = t.bounce(S, I, stack, cnt, G);
+    if (!live) {
+      want_replay = !t.replay && any_nonzero(t.L[0]);
+      if (!want_replay) ++s;
+    }
   }
 }
 
