@@ -148,7 +148,7 @@ __global__ void batch_supergrid_kernel(const BatchJobDev *__restrict__ jobs, con
 
 // ---------------------------------------------------------------- 2. render
 template <int NS, int INTEG, int F>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void batch_render_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void batch_render_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
                                     const int64_t *__restrict__ ends, int n, int64_t total, unsigned *queue,
                                     int grab) {
   extern __shared__ int lds_stack[];
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kElemBlock) void batch_zero_grad_kernel(BatchJobDev
 }
 
 template <int INTEG, int F>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void batch_adjoint_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void batch_adjoint_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
                                      const int64_t *__restrict__ ends, int n, int64_t total, unsigned *queue,
                                      int grab) {
   extern __shared__ int lds_stack[];

@@ -37,10 +37,8 @@ __device__ inline void flush_counters(const SceneView &S, LocalCounters &cnt) {
   }
 }
 
-// Three waves per SIMD: measured faster than two without spills or four with
-// them for the volume trackers, which are bound by memory latency.
 template <int NS, int INTEG, int F>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) void render_kernel(
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void render_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
                                                         int spp, int spt, int chunk_log2, float *img_old,
                                                         float *img_new) {
@@ -104,7 +102,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) 
 // same random numbers that scatters d(log weight)/d(theta) times the radiance
 // still to come into the parameter gradients. dimage/dL = 1/spp (box filter).
 template <int INTEG, int F>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(3))) void adjoint_kernel(
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void adjoint_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
                                                          int spp, int spt, int chunk_log2,
                                                          const float *__restrict__ grad_img, GradTargets G) {

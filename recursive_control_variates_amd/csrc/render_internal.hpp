@@ -27,6 +27,13 @@ inline void with_features(int f, Fn &&fn) {
   }
 }
 
+// Waves per SIMD of the render kernels: the trackers are bound by memory
+// latency, so occupancy pays while spills stay small. Measured: the volume
+// path tracer (<= 128 VGPRs, few spills) is fastest at four; volpathsimple and
+// the surface path tracer, whose state is larger, at three (<= 168 VGPRs).
+template <int INTEG>
+constexpr int kWavesPerEu = INTEG == RCV_INTEGRATOR_VOLPATH ? 4 : 3;
+
 // Splits the spp camera samples of a pixel over 2^chunk_log2 threads (at
 // most spt samples each): a user-given samples_per_thread must divide spp
 // into a power of two <= 64 chunks; otherwise chunks double, up to the
