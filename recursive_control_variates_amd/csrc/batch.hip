@@ -171,8 +171,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
       V3 acc[NS];
 #pragma unroll
       for (int q = 0; q < NS; ++q) acc[q] = v3(0.f);
-      LocalCounters cnt;
-      cnt.zero();
+      LocalCounters cnt{nullptr};
       const SensorInfo &sen = S.sensors[J.sensor];
       const uint32_t seed = iter_seed(J);
       if (pixel < J.n_pix) {
@@ -316,8 +315,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
       const int64_t local = u0 + lane - J.unit_begin_g;
       const int64_t pixel = local >> J.cl2_g;
       const int chunk = int(local & ((1 << J.cl2_g) - 1));
-      LocalCounters cnt;
-      cnt.zero();
+      LocalCounters cnt{nullptr};
       if (pixel < J.n_pix) {
         const SensorInfo &sen = S.sensors[J.sensor];
         const int px = int(pixel % J.film_w), py = int(pixel / J.film_w);

@@ -52,12 +52,12 @@ __device__ void path_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, 
   float prev_pdf = 1.0f;
   bool prev_delta = true;
   const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
-  cnt.c[kCntPaths]++;
+  cnt.add(kCntPaths, 1);
 
   while (true) {
     Hit h;
     trace<false>(S, ray, h, stack);
-    cnt.c[kCntTraversals]++;
+    cnt.add(kCntTraversals, 1);
     const SurfInt si = make_si(S, ray, h);
 
     // ---- direct emission (:281-290)
@@ -114,7 +114,7 @@ __device__ void path_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, 
         if (active_em && any_nonzero(em_w)) {
           const Ray sr = spawn_ray_to(si, ds.p);
           Hit sh;
-          cnt.c[kCntTraversals]++;
+          cnt.add(kCntTraversals, 1);
           if (trace<true>(S, sr, sh, stack)) em_w = v3(0.f);
         }
         if (active_em && any_nonzero(em_w)) {
@@ -133,7 +133,7 @@ __device__ void path_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, 
           const V3 r = rho[NS - 1];
           const V3 gv = v3(r.x > 0.f ? dl.x / r.x : 0.f, r.y > 0.f ? dl.y / r.y : 0.f, r.z > 0.f ? dl.z / r.z : 0.f);
           scatter_texture(g, tf, gv);
-          cnt.c[kCntGradAtomics] += 12;
+          cnt.add(kCntGradAtomics, 12);
         }
       }
 #pragma unroll
@@ -199,7 +199,7 @@ __device__ void path_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, 
     active_next = active_next && (!rr_active || urr < rr_prob);
     if (si.valid) {
       depth += 1;
-      cnt.c[kCntSurfaceHits]++;
+      cnt.add(kCntSurfaceHits, 1);
     }
     if (!active_next) break;
   }

@@ -26,17 +26,6 @@ namespace {
 constexpr int kMaxBlock = 256;      // launch bound of the render kernels
 constexpr int kDefaultBlock = 256;  // threads per block when the descriptor says 0
 
-__device__ inline void flush_counters(const SceneView &S, LocalCounters &cnt) {
-  if (!S.counters) return;
-#pragma unroll
-  for (int i = 0; i < kNumCounters; ++i) {
-    unsigned v = cnt.c[i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&S.counters[i], (unsigned long long)v);
-  }
-}
-
 template <int NS, int INTEG, int F>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void render_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
@@ -53,8 +42,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
   V3 acc[NS];
 #pragma unroll
   for (int k = 0; k < NS; ++k) acc[k] = v3(0.f);
-  LocalCounters cnt;
-  cnt.zero();
+  LocalCounters cnt{S.counters};
   if (pixel < n_pix) {
     const int px = int(pixel % film_w), py = int(pixel / film_w);
     const int s_end = sample_end(chunk, spp, chunk_log2);
@@ -95,7 +83,6 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
       o1[2] = acc[NS - 1].z / inv;
     }
   }
-  flush_counters(S, cnt);
 }
 
 // Path-replay adjoint (NEW state): primal pass for L, then a replay with the
@@ -114,8 +101,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t pixel = tid >> chunk_log2;
   const int chunk = int(tid & ((1 << chunk_log2) - 1));
-  LocalCounters cnt;
-  cnt.zero();
+  LocalCounters cnt{S.counters};
   if (pixel < n_pix) {
     const int px = int(pixel % film_w), py = int(pixel / film_w);
     const float inv = float(spp);
@@ -150,7 +136,6 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
       }
     }
   }
-  flush_counters(S, cnt);
 }
 
 __global__ void zero_counters_kernel(unsigned long long *c) {

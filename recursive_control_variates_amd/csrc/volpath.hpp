@@ -32,11 +32,17 @@ struct GradTargets {
   float *g[kMaxParams];
 };
 
+// Event counters of a launch (scene counting mode): each call adds n per
+// active lane of the wavefront with one atomic from its first active lane.
+// With a null pointer (the timed launches) every call is a no-op, so counting
+// costs no registers there.
 struct LocalCounters {
-  unsigned c[kNumCounters];
-  __device__ void zero() {
-#pragma unroll
-    for (int i = 0; i < kNumCounters; ++i) c[i] = 0;
+  unsigned long long *g;
+  __device__ void add(int k, unsigned n) const {
+    if (!g) return;
+    const uint64_t m = __ballot(1);
+    if (__lane_id() == unsigned(__ffsll((unsigned long long)m) - 1))
+      atomicAdd(g + k, (unsigned long long)(__popcll(m) * n));
   }
 };
 
@@ -304,7 +310,7 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
     if (needs_int) {
       Hit h;
       trace<false>(S, ray, h, stack);
-      cnt.c[kCntTraversals]++;
+      cnt.add(kCntTraversals, 1);
       si = make_si(S, ray, h);
       needs_int = false;
     }
@@ -316,7 +322,7 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
     if (active_medium) {
       const float u = rng.next_1d();
       sample_medium<NS, F>(S, medium, ray, u, mev);
-      cnt.c[kCntNeeSteps]++;
+      cnt.add(kCntNeeSteps, 1);
       if (si.t < mev.t) {
         mev.valid = false;
         mev.t = kInf;
@@ -337,7 +343,7 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
         float *g = G->g[M.sigma_param];
         if (g && mev.sn[NS - 1] != 0.0f) {
           scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
-          cnt.c[kCntGradAtomics] += 8;
+          cnt.add(kCntGradAtomics, 8);
         }
       }
     }
@@ -416,7 +422,7 @@ struct VolPath {
     medium = -1;
     replay = ADJ && is_replay;
     (void)rng.next_1d();  // colour channel for free-flight sampling (:171-173)
-    cnt.c[kCntPaths]++;
+    cnt.add(kCntPaths, 1);
   }
 
   // One bounce; false once the path has ended.
@@ -449,11 +455,11 @@ struct VolPath {
     if (active_medium) {
       const float u = rng.next_1d();
       sample_medium<NS, F>(S, medium, ray, u, mev);
-      cnt.c[kCntCollisions]++;
+      cnt.add(kCntCollisions, 1);
       if (needs_int) {
         Hit h;
         trace<false>(S, ray, h, stack);
-        cnt.c[kCntTraversals]++;
+        cnt.add(kCntTraversals, 1);
         si = make_si(S, ray, h);
       }
       needs_int = false;
@@ -480,7 +486,7 @@ struct VolPath {
       const float un = rng.next_1d();
       act_null = un >= p;
       act_scatter = !act_null;
-      if (act_null) cnt.c[kCntNullEvents]++;
+      if (act_null) cnt.add(kCntNullEvents, 1);
       if (NS == 2 && act_null) {
         const float den = mev.sn[1] + mev.sn[0];
         w[1] = w[1] * (2.0f * mev.sn[1] / den);
@@ -498,7 +504,7 @@ struct VolPath {
       si.t = si.t - mev.t;
     }
     if (act_scatter) {
-      cnt.c[kCntRealScatter]++;
+      cnt.add(kCntRealScatter, 1);
       if (NS == 2) {
         const float den = mev.st[1] + mev.st[0];
         w[1] = w[1] * (2.0f * mev.ss[1] / den);
@@ -523,7 +529,7 @@ struct VolPath {
         } else {
           scatter_grid(gd, mev.foot, lr * M.scale / mev.st[0]);
         }
-        cnt.c[kCntGradAtomics] += 8;
+        cnt.add(kCntGradAtomics, 8);
       }
       if ((F & kFeatAlbedoGrid) && act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
         const V3 dl = adj.dL * adj.Lrem;
@@ -532,7 +538,7 @@ struct VolPath {
         GridFoot fa;
         grid_footprint(S.params[M.albedo_param], M, mev.p, fa);
         scatter_grid3(G->g[M.albedo_param], fa, ga);
-        cnt.c[kCntGradAtomics] += 24;
+        cnt.add(kCntGradAtomics, 24);
       }
     }
 
@@ -554,7 +560,7 @@ struct VolPath {
     if (active_surface && needs_int) {
       Hit h;
       trace<false>(S, ray, h, stack);
-      cnt.c[kCntTraversals]++;
+      cnt.add(kCntTraversals, 1);
       si = make_si(S, ray, h);
     }
     if (active_surface) {
@@ -677,7 +683,7 @@ struct VolPath {
         depth += 1;
         last_p = si.p;
         last_pdf = bs.pdf;
-        cnt.c[kCntSurfaceHits]++;
+        cnt.add(kCntSurfaceHits, 1);
       }
       if (!bs.is_null) specular_chain = false;  // diffuse lobe is smooth (:376-377)
       const ShapeInfo sh = S.shapes[si.shape];

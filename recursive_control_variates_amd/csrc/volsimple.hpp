@@ -53,14 +53,14 @@ __device__ inline void simple_real_collision(const SceneView &S, int medium, Ray
   float run_t = 0.0f;
   while (true) {
     sample_medium<1, F>(S, medium, ray, rng.next_1d(), ev);
-    cnt.c[kCntCollisions]++;
+    cnt.add(kCntCollisions, 1);
     if (!ev.valid) return;
     const float r = ev.maj != 0.0f ? ev.st[0] / ev.maj : 0.0f;
     if (!(rng.next_1d() >= r)) {
       ev.t += run_t;
       return;
     }
-    cnt.c[kCntNullEvents]++;
+    cnt.add(kCntNullEvents, 1);
     ray.o = ev.p;
     ray.maxt -= ev.t;
     run_t += ev.t;
@@ -78,7 +78,7 @@ __device__ inline float simple_ratio_tracking(const SceneView &S, int medium, Ra
   while (true) {
     MediumEv<1> ev;
     sample_medium<1, F>(S, medium, ray, rng.next_1d(), ev);
-    cnt.c[kCntNeeSteps]++;
+    cnt.add(kCntNeeSteps, 1);
     if (!ev.valid) break;
     const float tr = ev.maj != 0.0f ? ev.sn[0] / ev.maj : ev.sn[0];
     if (GRAD && tr > 0.0f) {
@@ -86,7 +86,7 @@ __device__ inline float simple_ratio_tracking(const SceneView &S, int medium, Ra
       float *g = G->g[M.sigma_param];
       if (g && adj != 0.0f) {
         scatter_grid(g, ev.foot, -adj * M.scale / ev.sn[0]);
-        cnt.c[kCntGradAtomics] += 8;
+        cnt.add(kCntGradAtomics, 8);
       }
     }
     T *= tr;
@@ -114,7 +114,7 @@ __device__ inline V3 simple_sample_emitter(const SceneView &S, int medium, V3 p,
   r.maxt = kInf;
   Hit h;
   trace<false>(S, r, h, stack);
-  cnt.c[kCntTraversals]++;
+  cnt.add(kCntTraversals, 1);
   const SurfInt si = make_si(S, r, h);
   if (!si.valid) return v3(0.f);
   return ev * simple_ratio_tracking<F, GRAD>(S, medium, r, si.t, rng, cnt, adj, G);
@@ -190,11 +190,11 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
   float last_pdf = start.last_pdf;
   SurfInt si = start.si;
   if (start.from_camera) {
-    cnt.c[kCntPaths]++;
+    cnt.add(kCntPaths, 1);
     (void)rng.next_1d();  // colour channel (:86)
     Hit h;
     trace<false>(S, ray, h, stack);
-    cnt.c[kCntTraversals]++;
+    cnt.add(kCntTraversals, 1);
     const SurfInt s0 = make_si(S, ray, h);
     escaped = !s0.valid;
     active = s0.valid;
@@ -202,7 +202,7 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
     if (active) {
       ray = spawn_ray(s0, ray.d);
       trace<false>(S, ray, h, stack);
-      cnt.c[kCntTraversals]++;
+      cnt.add(kCntTraversals, 1);
       si = make_si(S, ray, h);
       active = si.valid;
       ray.maxt = si.valid ? si.t : kInf;
@@ -268,13 +268,13 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
           float *gd = G->g[M.sigma_param];
           if (gd) {
             scatter_grid(gd, mev.foot, mis * dot(dl, albedo) * M.scale / mev.st[0]);
-            cnt.c[kCntGradAtomics] += 8;
+            cnt.add(kCntGradAtomics, 8);
           }
           if ((F & kFeatAlbedoGrid) && M.albedo_param >= 0 && G->g[M.albedo_param]) {
             GridFoot fa;
             grid_footprint(S.params[M.albedo_param], M, mev.p, fa);
             scatter_grid3(G->g[M.albedo_param], fa, dl * mis);
-            cnt.c[kCntGradAtomics] += 24;
+            cnt.add(kCntGradAtomics, 24);
           }
         }
       }
@@ -290,7 +290,7 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
             GridFoot f;
             grid_footprint(P, M, ray_at(ray, tj), f);
             scatter_grid(gd, f, -lr * M.scale * interval * 0.25f);
-            cnt.c[kCntGradAtomics] += 8;
+            cnt.add(kCntGradAtomics, 8);
           }
         }
       }
@@ -299,7 +299,7 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
     thr = thr * albedo;
     if (did_scatter) {
       depth += 1;
-      cnt.c[kCntRealScatter]++;
+      cnt.add(kCntRealScatter, 1);
     }
     active = did_scatter && depth < max_depth;
 
@@ -339,7 +339,7 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
       ray.maxt = kInf;
       Hit h;
       trace<false>(S, ray, h, stack);
-      cnt.c[kCntTraversals]++;
+      cnt.add(kCntTraversals, 1);
       si = make_si(S, ray, h);
       ray.maxt = si.valid ? si.t : kInf;
     }
@@ -366,7 +366,7 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
     ray.maxt = kInf;
     Hit h;
     trace<false>(S, ray, h, stack);
-    cnt.c[kCntTraversals]++;
+    cnt.add(kCntTraversals, 1);
     const SurfInt se = make_si(S, ray, h);
     const int em = si_emitter(S, se);
     if (em >= 0 && !(depth <= 0 && I.hide_emitters)) {
@@ -433,11 +433,11 @@ __device__ void simple_drt(const SceneView &S, const IntegArgs &I, int medium, R
   float *gd = G->g[M.sigma_param];
   if (gd && weight != 0.0f) {
     scatter_grid(gd, foot, dot(a, alb) * M.scale);
-    cnt.c[kCntGradAtomics] += 8;
+    cnt.add(kCntGradAtomics, 8);
   }
   if ((F & kFeatAlbedoGrid) && M.albedo_param >= 0 && G->g[M.albedo_param] && weight != 0.0f) {
     scatter_grid3(G->g[M.albedo_param], fa, a * st);
-    cnt.c[kCntGradAtomics] += 24;
+    cnt.add(kCntGradAtomics, 24);
   }
 }
 
@@ -462,7 +462,7 @@ __device__ V3 simple_recursive_li(const SceneView &S, const IntegArgs &I, int me
   rec.maxt = kInf;
   Hit h;
   trace<false>(S, rec, h, stack);
-  cnt.c[kCntTraversals]++;
+  cnt.add(kCntTraversals, 1);
   SimpleStart st;
   st.from_camera = false;
   st.depth = depth + 1;
