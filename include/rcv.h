@@ -151,6 +151,14 @@ int rcv_scene_add_mesh(rcv_scene *scene, const float *vertices, const float *uv,
 int rcv_scene_add_sphere(rcv_scene *scene, const float center[3], float radius,
                          int32_t bsdf, int32_t interior, int32_t exterior,
                          int32_t *out_id);
+/* Oriented box: the image of [-1, 1]^3 under the affine map `to_world`
+ * (row-major 3x4), i.e. Mitsuba's 'cube' shape, which the reference's volume
+ * scenes use as the null-BSDF medium boundary (experiments.py:74-79). Traced
+ * as one analytic primitive; hit points, outward face normals and the
+ * per-face uv layout are those of the cube's 12-triangle mesh. Not an area
+ * emitter (an emitting cube is added as a mesh). */
+int rcv_scene_add_box(rcv_scene *scene, const float to_world[12], int32_t bsdf,
+                      int32_t interior, int32_t exterior, int32_t *out_id);
 /* A (batch) sensor: n_views perspective cameras side by side on one film of
  * film_width x film_height; view k covers columns [k*w/n, (k+1)*w/n).
  * to_world: n_views row-major 4x4 camera-to-world matrices (columns: left,

@@ -88,6 +88,7 @@ SIGNATURES = {
     "rcv_scene_add_mesh": [_P, C.POINTER(_F), C.POINTER(_F), _I, C.POINTER(_I), _I, _I, _I, _I, _I,
                            C.POINTER(_I)],
     "rcv_scene_add_sphere": [_P, C.POINTER(_F), _F, _I, _I, _I, C.POINTER(_I)],
+    "rcv_scene_add_box": [_P, C.POINTER(_F), _I, _I, _I, C.POINTER(_I)],
     "rcv_scene_add_sensor": [_P, _I, _I, _I, C.POINTER(_F), C.POINTER(_F), C.POINTER(_I)],
     "rcv_scene_commit": [_P, _P],
     "rcv_scene_check_twostate": [_P, _I, C.POINTER(_I), _I, C.POINTER(_I), _I],

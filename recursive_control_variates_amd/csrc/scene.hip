@@ -187,8 +187,9 @@ int Scene::upload(hipStream_t stream) {
   // ---- primitives
   const int n_tris = int(tri_v.size() / 9);
   const int n_sph = int(sph.size() / 4);
+  const int n_box = int(box_shape.size());
   std::vector<BuildPrim> bp;
-  bp.reserve(n_tris + n_sph);
+  bp.reserve(n_tris + n_sph + n_box);
   for (int t = 0; t < n_tris; ++t) {
     BuildPrim p;
     for (int k = 0; k < 3; ++k) p.box.grow(&tri_v[9 * t + 3 * k]);
@@ -207,6 +208,52 @@ int Scene::upload(hipStream_t stream) {
     }
     p.id = n_tris + s;
     bp.push_back(p);
+  }
+  std::vector<BoxPrim> boxes(std::max(1, n_box));
+  for (int b = 0; b < n_box; ++b) {
+    const float *m = &box_xf[12 * b];
+    BuildPrim p;
+    for (int c = 0; c < 8; ++c) {
+      const float q[3] = {c & 1 ? 1.f : -1.f, c & 2 ? 1.f : -1.f, c & 4 ? 1.f : -1.f};
+      float w[3];
+      for (int r = 0; r < 3; ++r) w[r] = m[4 * r] * q[0] + m[4 * r + 1] * q[1] + m[4 * r + 2] * q[2] + m[4 * r + 3];
+      p.box.grow(w);
+    }
+    for (int a = 0; a < 3; ++a) p.c[a] = m[4 * a + 3];
+    p.id = n_tris + n_sph + b;
+    bp.push_back(p);
+    // inverse of the linear part (cofactors, in double) and of the offset
+    double A[3][3], I[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) A[r][c] = m[4 * r + c];
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                       A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                       A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+        I[r][c] = (A[r1][c1] * A[r2][c2] - A[r1][c2] * A[r2][c1]) / det;
+      }
+    BoxPrim &B = boxes[b];
+    for (int r = 0; r < 3; ++r) {
+      double off = 0;
+      for (int c = 0; c < 3; ++c) {
+        B.inv[4 * r + c] = float(I[r][c]);
+        off -= I[r][c] * m[4 * c + 3];
+      }
+      B.inv[4 * r + 3] = float(off);
+    }
+    // outward world normal of face (axis k, sign sg): inverse-transpose image
+    // of the object normal, i.e. sg times row k of the inverse
+    for (int f = 0; f < 6; ++f) {
+      const int k = f / 2;
+      const double sg = (f & 1) ? -1.0 : 1.0;
+      const double n[3] = {sg * I[k][0], sg * I[k][1], sg * I[k][2]};
+      const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      for (int a = 0; a < 3; ++a) B.n[3 * f + a] = float(n[a] / len);
+    }
+    B.shape = box_shape[b];
+    B.pad = 0;
   }
   RCV_REQUIRE(!bp.empty(), "rcv_scene_commit: the scene has no shapes");
   Builder b(bp);
@@ -300,6 +347,7 @@ int Scene::upload(hipStream_t stream) {
   const size_t o_pid = put(prim_ids.data(), prim_ids.size() * 4);
   const size_t o_sph = put(spheres.data(), spheres.size() * sizeof(float4));
   const size_t o_sphs = put(sphere_shape_v.data(), sphere_shape_v.size() * 4);
+  const size_t o_box = put(boxes.data(), boxes.size() * sizeof(BoxPrim));
   const size_t o_shapes = put(shapes.data(), shapes.size() * sizeof(ShapeInfo));
   const size_t o_bsdfs = put(bsdfs.data(), bsdfs.size() * sizeof(BsdfInfo));
   const size_t o_em = put(emitters.data(), emitters.size() * sizeof(EmitterInfo));
@@ -325,6 +373,7 @@ int Scene::upload(hipStream_t stream) {
   v.prim_ids = reinterpret_cast<const int32_t *>(base + o_pid);
   v.spheres = reinterpret_cast<const float4 *>(base + o_sph);
   v.sphere_shape = reinterpret_cast<const int32_t *>(base + o_sphs);
+  v.boxes = reinterpret_cast<const BoxPrim *>(base + o_box);
   v.shapes = reinterpret_cast<const ShapeInfo *>(base + o_shapes);
   v.bsdfs = reinterpret_cast<const BsdfInfo *>(base + o_bsdfs);
   v.emitters = reinterpret_cast<const EmitterInfo *>(base + o_em);
@@ -338,6 +387,7 @@ int Scene::upload(hipStream_t stream) {
   counters = reinterpret_cast<unsigned long long *>(base + o_cnt);
   v.counters = nullptr;
   v.n_tris = n_tris;
+  v.n_sph = n_sph;
   v.n_emitters = int(emitters.size());
   v.env_emitter = -1;
   for (int i = 0; i < int(emitters.size()); ++i)
@@ -665,6 +715,26 @@ extern "C" int rcv_scene_add_sphere(rcv_scene *s, const float center[3], float r
   for (int k = 0; k < 3; ++k) s->sph.push_back(center[k]);
   s->sph.push_back(radius);
   s->sph_shape.push_back(shape);
+  s->shapes.push_back(ShapeInfo{bsdf, -1, interior, exterior});
+  if (out_id) *out_id = shape;
+  return RCV_OK;
+}
+
+extern "C" int rcv_scene_add_box(rcv_scene *s, const float to_world[12], int32_t bsdf, int32_t interior,
+                                 int32_t exterior, int32_t *out_id) {
+  REQUIRE_OPEN(s, "rcv_scene_add_box");
+  RCV_REQUIRE(to_world, "rcv_scene_add_box: null transform");
+  for (int i = 0; i < 12; ++i) RCV_REQUIRE(std::isfinite(to_world[i]), "rcv_scene_add_box: non-finite transform");
+  const float *m = to_world;
+  const double det = double(m[0]) * (double(m[5]) * m[10] - double(m[6]) * m[9]) -
+                     double(m[1]) * (double(m[4]) * m[10] - double(m[6]) * m[8]) +
+                     double(m[2]) * (double(m[4]) * m[9] - double(m[5]) * m[8]);
+  RCV_REQUIRE(det != 0.0, "rcv_scene_add_box: singular transform");
+  int rc = check_refs(s, bsdf, -1, interior, exterior, "rcv_scene_add_box");
+  if (rc) return rc;
+  const int shape = int(s->shapes.size());
+  s->box_xf.insert(s->box_xf.end(), to_world, to_world + 12);
+  s->box_shape.push_back(shape);
   s->shapes.push_back(ShapeInfo{bsdf, -1, interior, exterior});
   if (out_id) *out_id = shape;
   return RCV_OK;

@@ -94,6 +94,56 @@ __device__ inline bool hit_triangle(const float4 *tris, int slot, const Ray &r, 
   return true;
 }
 
+// Ray/oriented-box test: slabs of [-1, 1]^3 in the box's object space, where
+// the ray keeps its parameter t. Returns the first crossing with t in (0,
+// tmax) -- the entry face from outside, the exit face from inside -- as face
+// 2k (+k) or 2k + 1 (-k). A NaN slab bound (ray parallel to and on a face
+// plane) leaves the interval unchanged, as in hit_box.
+__device__ inline bool hit_boxprim(const BoxPrim &B, const Ray &r, float tmax, float &t, int &face) {
+  const float *m = B.inv;
+  const float o[3] = {fmaf(m[0], r.o.x, fmaf(m[1], r.o.y, fmaf(m[2], r.o.z, m[3]))),
+                      fmaf(m[4], r.o.x, fmaf(m[5], r.o.y, fmaf(m[6], r.o.z, m[7]))),
+                      fmaf(m[8], r.o.x, fmaf(m[9], r.o.y, fmaf(m[10], r.o.z, m[11])))};
+  const float d[3] = {fmaf(m[0], r.d.x, fmaf(m[1], r.d.y, m[2] * r.d.z)),
+                      fmaf(m[4], r.d.x, fmaf(m[5], r.d.y, m[6] * r.d.z)),
+                      fmaf(m[8], r.d.x, fmaf(m[9], r.d.y, m[10] * r.d.z))};
+  float t0 = -kInf, t1 = kInf;
+  int f0 = -1, f1 = -1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float inv = 1.0f / d[k];
+    float ta = (-1.0f - o[k]) * inv, tb = (1.0f - o[k]) * inv;
+    int fa = 2 * k + 1, fb = 2 * k;
+    if (ta > tb) {
+      const float tmp = ta;
+      ta = tb;
+      tb = tmp;
+      fa = 2 * k;
+      fb = 2 * k + 1;
+    }
+    if (ta > t0) {
+      t0 = ta;
+      f0 = fa;
+    }
+    if (tb < t1) {
+      t1 = tb;
+      f1 = fb;
+    }
+  }
+  if (!(t0 <= t1)) return false;
+  if (t0 > 0.0f && t0 < tmax && f0 >= 0) {
+    t = t0;
+    face = f0;
+    return true;
+  }
+  if (t1 > 0.0f && t1 < tmax && f1 >= 0) {
+    t = t1;
+    face = f1;
+    return true;
+  }
+  return false;
+}
+
 // Ray/sphere test in the numerically robust form of Haines et al., "Precision
 // improvements for ray/sphere intersection" (Ray Tracing Gems, 2019).
 __device__ inline bool hit_sphere(float4 s, const Ray &r, float tmax, float &t) {
@@ -173,10 +223,15 @@ __device__ inline bool trace(const SceneView &S, const Ray &r, Hit &h, const Lds
         const int pid = S.prim_ids[slot];
         float t, b1 = 0.f, b2 = 0.f;
         bool hit;
-        if (pid < S.n_tris)
+        if (pid < S.n_tris) {
           hit = hit_triangle(S.tris, slot, r, tbest, t, b1, b2);
-        else
+        } else if (pid < S.n_tris + S.n_sph) {
           hit = hit_sphere(S.spheres[pid - S.n_tris], r, tbest, t);
+        } else {
+          int face = 0;
+          hit = hit_boxprim(S.boxes[pid - S.n_tris - S.n_sph], r, tbest, t, face);
+          b1 = float(face);
+        }
         if (hit) {
           tbest = t;
           h.t = t;
@@ -244,6 +299,25 @@ __device__ inline SurfInt make_si(const SceneView &S, const Ray &r, const Hit &h
     const float b0 = 1.0f - h.b1 - h.b2;
     si.u = uv0.x * b0 + uv1.x * h.b1 + uv2.x * h.b2;
     si.v = uv0.y * b0 + uv1.y * h.b1 + uv2.y * h.b2;
+  } else if (pid >= S.n_tris + S.n_sph) {
+    // box: the face's plane point and normal; uv as the 'cube' mesh lays it
+    // out per face (two triangles per face, corners (0,0) (1,0) (1,1) (0,1))
+    const BoxPrim &B = S.boxes[pid - S.n_tris - S.n_sph];
+    const int face = int(h.b1);
+    si.p = ray_at(r, h.t);
+    si.n = v3(B.n[3 * face], B.n[3 * face + 1], B.n[3 * face + 2]);
+    si.shape = B.shape;
+    const float *m = B.inv;
+    const float qx = 0.5f * (fmaf(m[0], si.p.x, fmaf(m[1], si.p.y, fmaf(m[2], si.p.z, m[3]))) + 1.0f);
+    const float qy = 0.5f * (fmaf(m[4], si.p.x, fmaf(m[5], si.p.y, fmaf(m[6], si.p.z, m[7]))) + 1.0f);
+    const float qz = 0.5f * (fmaf(m[8], si.p.x, fmaf(m[9], si.p.y, fmaf(m[10], si.p.z, m[11]))) + 1.0f);
+    const int k = face >> 1;
+    const bool neg = face & 1;
+    // +x: (y, z)  -x: (z, y)  +y: (z, x)  -y: (x, z)  +z: (x, y)  -z: (y, x)
+    const float a0 = k == 0 ? qy : (k == 1 ? qz : qx);
+    const float a1 = k == 0 ? qz : (k == 1 ? qx : qy);
+    si.u = neg ? a1 : a0;
+    si.v = neg ? a0 : a1;
   } else {
     const int s = pid - S.n_tris;
     const float4 sp = S.spheres[s];

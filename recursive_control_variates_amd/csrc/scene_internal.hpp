@@ -28,6 +28,8 @@ struct Scene {
   std::vector<int32_t> tri_shape;
   std::vector<float> sph;       // center xyz, radius
   std::vector<int32_t> sph_shape;
+  std::vector<float> box_xf;    // per box: 3x4 object-to-world map, row-major
+  std::vector<int32_t> box_shape;
 
   // after commit
   bool committed = false;

@@ -38,6 +38,17 @@ struct BVHNode {
   int32_t count;  // > 0: leaf with `count` primitives
 };
 
+// Oriented box primitive: the image of [-1, 1]^3 under an affine map (the
+// 'cube' shape, traced as one primitive instead of 12 triangles). inv is the
+// world-to-object map (3x4, row-major); n the outward unit world normals of
+// the faces +x, -x, +y, -y, +z, -z.
+struct BoxPrim {
+  float inv[12];
+  float n[18];
+  int32_t shape;
+  int32_t pad;
+};
+
 struct ShapeInfo {
   int32_t bsdf;      // -1: none
   int32_t emitter;   // -1: none
@@ -106,9 +117,10 @@ struct SceneView {
   const BVHNode *nodes;
   const float4 *tris;        // 3 float4 per triangle slot
   const float2 *tri_uv;      // 3 per triangle slot
-  const int32_t *prim_ids;   // slot -> primitive id (>= n_tris: sphere)
+  const int32_t *prim_ids;   // slot -> primitive id: triangles, then spheres, then boxes
   const float4 *spheres;     // center.xyz, radius
   const int32_t *sphere_shape;
+  const BoxPrim *boxes;
   const ShapeInfo *shapes;
   const BsdfInfo *bsdfs;
   const EmitterInfo *emitters;
@@ -120,6 +132,7 @@ struct SceneView {
   const float *supergrids;   // per-medium supergrid majorants (two states)
   unsigned long long *counters;  // kNumCounters or null
   int32_t n_tris;
+  int32_t n_sph;
   int32_t n_emitters;
   int32_t env_emitter;  // -1 if none
   int32_t n_media;

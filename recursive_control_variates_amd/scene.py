@@ -103,6 +103,9 @@ class Scene:
                 if s.kind == "sphere":
                     N.call("rcv_scene_add_sphere", h, N.f32_array(s.center), C.c_float(s.radius), s.bsdf,
                            s.interior, s.exterior, C.byref(out))
+                elif s.box is not None:
+                    N.call("rcv_scene_add_box", h, N.f32_array(np.asarray(s.box, np.float32).reshape(12)), s.bsdf,
+                           s.interior, s.exterior, C.byref(out))
                 else:
                     v = np.ascontiguousarray(s.vertices, np.float32)
                     f = np.ascontiguousarray(s.faces, np.int32)

@@ -97,6 +97,7 @@ class ShapeDesc:
     faces: Optional[np.ndarray] = None      # (m, 3) int32
     center: Optional[tuple] = None
     radius: float = 0.0
+    box: Optional[np.ndarray] = None        # 'cube': (3, 4) float32 object-to-world map of [-1,1]^3
 
 
 @dataclass
@@ -381,8 +382,11 @@ class _Loader:
         vw = tw.transform_point(v).astype(np.float32)
         if np.linalg.det(tw.matrix[:3, :3]) < 0:
             f = f[:, ::-1].copy()
+        # a non-emitting cube is also described as an analytic box (the GPU
+        # traces it as one primitive; the triangles describe the same surface)
+        box = np.ascontiguousarray(tw.matrix[:3, :4], np.float32) if t == "cube" and emitter < 0 else None
         self.desc.shapes.append(ShapeDesc(name, "mesh", bsdf, emitter, interior, exterior,
-                                          vertices=vw, uv=uv, faces=np.ascontiguousarray(f, np.int32)))
+                                          vertices=vw, uv=uv, faces=np.ascontiguousarray(f, np.int32), box=box))
 
     # ---- sensors
     def _sensor(self, name, d):

@@ -27,7 +27,9 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(lib, s), s
     assert lib.rcv_version() >= 1
-    # every declared entry point has a ctypes signature in the binding
+    # every declared entry point has a ctypes signature in the binding (the
+    # batch entry points register theirs when the batch module loads)
+    import recursive_control_variates_amd.batch  # noqa: F401
     for s in syms:
         assert s in N.SIGNATURES or s in ("rcv_version", "rcv_last_error"), s
 
@@ -180,3 +182,27 @@ def test_adjust_majorant_res_factor_rule():
         adjust_majorant_res_factor(s, (side, side + 5, side + 9, 1))
         assert s.majorant_resolution_factor(0) == k, (side, k)
         assert s.calls == ([] if k == 0 else [k])
+
+
+def test_cube_shapes_carry_an_analytic_box():
+    # a non-emitting cube is uploaded as one oriented box (its triangles stay
+    # in the description for the oracle); emitting cubes and other shapes
+    # stay meshes
+    from recursive_control_variates_amd.scene_desc import describe
+    from recursive_control_variates_amd.transform import T
+    tw = T.translate([0.5, 0.5, 0.5]).rotate([0, 1, 0], 30).scale([0.5, 0.25, 0.5])
+    d = {"type": "scene",
+         "sensor": {"type": "perspective", "film": {"type": "hdrfilm", "width": 4, "height": 4}},
+         "a": {"type": "cube", "to_world": tw, "bsdf": {"type": "null"}},
+         "b": {"type": "cube", "emitter": {"type": "area", "radiance": {"type": "rgb", "value": 1.0}}},
+         "c": {"type": "rectangle"}}
+    desc = describe(d)
+    by_name = {s.name: s for s in desc.shapes}
+    box = by_name["a"].box
+    assert box is not None and box.shape == (3, 4)
+    np.testing.assert_allclose(box, np.asarray(tw.matrix, np.float64)[:3, :4], rtol=1e-6, atol=1e-7)
+    # the box maps the mesh's object-space corners onto its world vertices
+    v = by_name["a"].vertices
+    obj = np.array([[1, -1, -1], [1, 1, -1], [1, 1, 1], [1, -1, 1]], np.float64)
+    np.testing.assert_allclose(obj @ box[:, :3].T.astype(np.float64) + box[:, 3], v[:4], rtol=1e-6, atol=1e-6)
+    assert by_name["b"].box is None and by_name["c"].box is None
