@@ -41,7 +41,8 @@ namespace rcv {
 
 constexpr int kWave = 64;
 constexpr int kElemBlock = 256;
-constexpr int kBatchMaxSpt = 8;  // camera samples per work unit (at most)
+constexpr int kBatchMaxSpt = 8;          // camera samples per work unit (at most)
+constexpr int64_t kBatchMinThreads = 0;  // the batch, not one job, fills the device
 
 struct BatchJobDev {
   SceneView view;       // params[param].data[] rotated on the device
@@ -477,8 +478,9 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
   J.spp_g = d->spp_grad;
   // finer work units than a single launch: a batch has parallelism to
   // spare, and short units keep the end of a persistent launch balanced
-  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp, J.spt, J.cl2, kBatchMaxSpt)) return rc;
-  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp_g, J.spt_g, J.cl2_g, kBatchMaxSpt)) return rc;
+  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp, J.spt, J.cl2, kBatchMaxSpt, kBatchMinThreads)) return rc;
+  if (int rc = plan_launch(&d->integrator, J.n_pix, J.spp_g, J.spt_g, J.cl2_g, kBatchMaxSpt, kBatchMinThreads))
+    return rc;
   J.seed = d->seed;
   J.param = d->param;
   J.n_param = int64_t(P.nx) * P.ny * P.nz * P.ch;

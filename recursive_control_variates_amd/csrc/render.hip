@@ -147,7 +147,8 @@ __global__ void zero_counters_kernel(unsigned long long *c) {
 
 static size_t stack_bytes(int block) { return size_t(kStackSize) * block * sizeof(int); }
 
-int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2, int max_spt) {
+int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2, int max_spt,
+                int64_t min_threads) {
   if (d && d->block_size != 0 && (d->block_size < 64 || d->block_size > kMaxBlock || d->block_size % 64 != 0))
     return fail(RCV_E_INVALID, "block_size %d must be 0 or a multiple of 64 in [64, %d]", d->block_size, kMaxBlock);
   int chunks;
@@ -161,7 +162,7 @@ int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, 
       return fail(RCV_E_INVALID, "spp / samples_per_thread = %d must be a power of two <= 64", chunks);
   } else {
     chunks = 1;
-    while (chunks < cap && (n_pix * chunks < (int64_t(1) << 19) || (max_spt > 0 && spp > max_spt * chunks)))
+    while (chunks < cap && (n_pix * chunks < min_threads || (max_spt > 0 && spp > max_spt * chunks)))
       chunks <<= 1;
   }
   spt = (spp + chunks - 1) / chunks;  // samples of the busiest thread

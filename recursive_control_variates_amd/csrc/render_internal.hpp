@@ -37,10 +37,11 @@ constexpr int kWavesPerEu = INTEG == RCV_INTEGRATOR_VOLPATH ? 4 : 3;
 // Splits the spp camera samples of a pixel over 2^chunk_log2 threads (at
 // most spt samples each): a user-given samples_per_thread must divide spp
 // into a power of two <= 64 chunks; otherwise chunks double, up to the
-// largest power of two <= min(spp, 64), while the launch has fewer than 2^19
-// threads or (max_spt > 0) a thread would trace more than max_spt samples.
+// largest power of two <= min(spp, 64), while the launch has fewer than
+// min_threads threads or (max_spt > 0) a thread would trace more than
+// max_spt samples.
 int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2,
-                int max_spt = 0);
+                int max_spt = 0, int64_t min_threads = int64_t(1) << 19);
 
 // One past the last sample of chunk c (chunk -1 ends at 0): chunk c traces
 // samples [sample_end(c - 1), sample_end(c)), an even split when the chunk

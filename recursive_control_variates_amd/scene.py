@@ -246,7 +246,7 @@ class Scene:
                 N.lib().rcv_scene_destroy(self._handle)
             except Exception:
                 pass
-            self._handle = C.c_void_p()
+            self._handle = None
 
     def __del__(self):
         self._destroy()

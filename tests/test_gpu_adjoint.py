@@ -160,3 +160,44 @@ def test_albedo_grid_adjoint_matches_finite_differences(cuda):
     fds, adj = np.array(fds), np.array(adj)
     se = np.sqrt(fds.var(ddof=1) / len(fds) + adj.var(ddof=1) / len(adj))
     assert abs(fds.mean() - adj.mean()) < 4 * se + 0.02 * abs(fds.mean()), (fds.mean(), adj.mean(), se)
+
+
+@pytest.mark.parametrize("which", ["sigma_t", "albedo"])
+def test_adjoint_equals_finite_difference_with_common_random_numbers(cuda, which):
+    # The two-state render of (theta + eps delta, theta - eps delta) and the
+    # path-replay adjoint at theta trace the same paths for the same seed: the
+    # central difference of the shared-path estimator is then the per-path
+    # derivative the adjoint scatters, up to O(eps^2) and float rounding --
+    # seed by seed, with no Monte Carlo tolerance (dense medium, paths long
+    # enough to reach max_depth).
+    announce(f"adjoint == FD with common random numbers ({which})")
+    from recursive_control_variates_amd.transform import T
+    rng = np.random.default_rng(21)
+    n = 6
+    dens = (0.3 + rng.uniform(0, 0.7, (n, n, n, 1))).astype(np.float32)
+    alb = rng.uniform(0.5, 0.95, (n, n, n, 3)).astype(np.float32)
+    alb_dict = {"type": "gridvolume", "data": alb, "to_world": T.scale(1.0)}
+    sc = rcv.load_dict(volume_box_scene(dens, albedo=alb_dict, scale=12.0, w=16, h=12, spp=8))
+    key = f"box.interior_medium.{which}.data"
+    base = dens if which == "sigma_t" else alb
+    gimg = torch.from_numpy(rng.uniform(0.5, 1.5, (12, 16, 3)).astype(np.float32)).cuda()
+    delta = rng.uniform(-1, 1, base.shape).astype(np.float32)
+    # the majorant (grid maximum) must not move, or free flights would differ
+    delta[base > 0.9 * base.max()] = 0.0
+    eps = 1e-3 * float(base.max())
+    ts = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 12, "rr_depth": 12})
+    plain = rcv.load_dict({"type": "prbvolpath", "max_depth": 12, "rr_depth": 12})
+    p = rcv.traverse(sc)
+    for s in range(3):
+        p[key] = torch.from_numpy(base + eps * delta).cuda()
+        p.update()
+        p[key] = torch.from_numpy(base - eps * delta).cuda()
+        p.update()
+        H, F = ts.render_twostates(sc, 0, 40 + s, 64)
+        fd = float(((H.double() - F.double()) * gimg.double()).sum()) / (2 * eps)
+        p[key] = torch.from_numpy(base).cuda().requires_grad_(True)
+        p.update()
+        p.update()
+        g = plain.adjoint(sc, p, gimg, 0, 40 + s, 64)[key]
+        ad = float((g.double() * torch.from_numpy(delta).cuda().double()).sum())
+        assert abs(fd - ad) <= 0.01 * abs(fd), (s, fd, ad)

@@ -87,15 +87,17 @@ def test_albedo_and_density_gradients_match_finite_differences(cuda):
             p.update()
             H, F = ts.render_twostates(sc, 0, 900 + s, 2048)
             fds.append(float(np.sum(gimg * (H - F).cpu().numpy()) / (2 * eps)))
+            # old = new = theta for every key: update() shifts only the keys
+            # assigned since the last update, so each restore assigns twice
             x = torch.from_numpy(base).cuda().requires_grad_(True)
-            p[key] = x
-            p.update()
-            p.update()
+            for _ in range(2):
+                p[key] = x
+                p.update()
             gr = simple.adjoint(sc, p, torch.from_numpy(gimg).cuda(), 0, 1900 + s, 2048)[key].cpu().numpy()
             adj.append(float(np.sum(gr * delta)))
-            p[key] = torch.from_numpy(base).cuda()
-            p.update()
-            p.update()
+            for _ in range(2):
+                p[key] = torch.from_numpy(base).cuda()
+                p.update()
         fds, adj = np.array(fds), np.array(adj)
         se = np.sqrt(fds.var(ddof=1) / len(fds) + adj.var(ddof=1) / len(adj))
         assert abs(fds.mean() - adj.mean()) < 4 * se + 0.03 * abs(fds.mean()), (key, fds.mean(), adj.mean(), se)
