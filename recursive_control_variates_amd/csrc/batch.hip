@@ -41,7 +41,7 @@ namespace rcv {
 
 constexpr int kWave = 64;
 constexpr int kElemBlock = 256;
-constexpr int kBatchMaxSpt = 8;          // camera samples per work unit (at most)
+constexpr int kBatchMaxSpt = 32;         // camera samples per work unit (at most)
 constexpr int64_t kBatchMinThreads = 0;  // the batch, not one job, fills the device
 
 struct BatchJobDev {
