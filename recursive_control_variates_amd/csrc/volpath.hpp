@@ -278,6 +278,18 @@ __device__ inline bool null_bsdf(const SceneView &S, int shape) {
   return b >= 0 && S.bsdfs[b].type == RCV_BSDF_NULL;
 }
 
+// The shadow-ray collisions of one NEE (point, sigma_n of the NEW state,
+// medium) recorded by the replay pass, so that the transmittance gradient can
+// be scattered once the NEE contribution is known without tracking the
+// shadow ray a second time; n counts every collision, and a shadow ray with
+// more than kNeeRecords of them is tracked again instead.
+constexpr int kNeeRecords = 32;
+struct NeeRecord {
+  int n;
+  float4 e[kNeeRecords];
+  int med[kNeeRecords];
+};
+
 // sample_emitter_twostates (plugins/twostatevolpath.py:387-479): emitter
 // direction without visibility, then ratio tracking of both transmittances.
 // GRAD = true replays the same random numbers and scatters
@@ -285,7 +297,8 @@ __device__ inline bool null_bsdf(const SceneView &S, int shape) {
 template <int NS, bool GRAD, int F>
 __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, bool ref_surface, int medium,
                                      Pcg32 &rng, V3 em[NS], DirSample &ds, const LdsStack &stack,
-                                     LocalCounters &cnt, float gscale, const GradTargets *G) {
+                                     LocalCounters &cnt, float gscale, const GradTargets *G,
+                                     NeeRecord *rec = nullptr) {
   const float u1 = rng.next_1d();
   const float u2 = rng.next_1d();
   const V3 ev = sample_emitter_direction(S, ref_p, u1, u2, ds);
@@ -345,6 +358,13 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
           scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
           cnt.add(kCntGradAtomics, 8);
         }
+      }
+      if (rec) {
+        if (rec->n < kNeeRecords) {
+          rec->e[rec->n] = make_float4(mev.p.x, mev.p.y, mev.p.z, mev.sn[NS - 1]);
+          rec->med[rec->n] = medium;
+        }
+        rec->n++;
       }
     }
     active_surface = (active_surface || escaped) && si.valid && !active_medium;
@@ -604,7 +624,10 @@ struct VolPath {
       V3 em[NS];
       DirSample ds;
       const Pcg32 rng_nee = rng;
-      nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G);
+      NeeRecord rec;
+      rec.n = 0;
+      nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G,
+                                  ADJ && replay ? &rec : nullptr);
       V3 nw[NS];
       float nee_pdf;
       if (active_e_surface) {
@@ -628,12 +651,27 @@ struct VolPath {
       if (ADJ && replay) {
         const float cs = dot(adj.dL, c[0]);
         if (cs != 0.0f && any_nonzero(c[0])) {
-          Pcg32 r2 = rng_nee;
-          V3 em2[NS];
-          DirSample ds2;
-          if constexpr (ADJ)
+          if (rec.n <= kNeeRecords) {
+            // d(log T)/d(density) at each recorded shadow-ray collision
+            for (int j = 0; j < rec.n; ++j) {
+              const MediumInfo &M = S.media[rec.med[j]];
+              float *g = G->g[M.sigma_param];
+              const float sn = rec.e[j].w;
+              if (g && sn != 0.0f) {
+                GridFoot f;
+                grid_footprint(S.params[M.sigma_param], M, v3(rec.e[j].x, rec.e[j].y, rec.e[j].z), f);
+                scatter_grid(g, f, -cs * M.scale / sn);
+                cnt.add(kCntGradAtomics, 8);
+              }
+            }
+          } else {
+            Pcg32 r2 = rng_nee;
+            V3 em2[NS];
+            DirSample ds2;
+            if constexpr (ADJ)
               nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs,
                                          G);
+          }
         }
         adj.Lrem -= c[0];
       }
