@@ -70,9 +70,13 @@ __device__ inline void simple_real_collision(const SceneView &S, int medium, Ray
 // Ratio-tracking transmittance over [0, tmax] of `ray` inside the medium
 // (estimate_transmittance, plugins/volpathsimple.py:450-491). GRAD: scatter
 // -adj * scale / sigma_n per tentative collision (the replay of :476-482).
+// rec (adjoint, first pass): the collisions that replay would scatter at
+// (point, sigma_n; sigma_n = 0 where it would not), so the caller can scatter
+// them once adj is known.
 template <int F, bool GRAD>
 __device__ inline float simple_ratio_tracking(const SceneView &S, int medium, Ray ray, float tmax, Pcg32 &rng,
-                                              LocalCounters &cnt, float adj, const GradTargets *G) {
+                                              LocalCounters &cnt, float adj, const GradTargets *G,
+                                              NeeRecord *rec = nullptr) {
   float T = 1.0f;
   ray.maxt = tmax;
   while (true) {
@@ -89,6 +93,13 @@ __device__ inline float simple_ratio_tracking(const SceneView &S, int medium, Ra
         cnt.add(kCntGradAtomics, 8);
       }
     }
+    if (rec) {
+      if (rec->n < kNeeRecords) {
+        rec->e[rec->n] = make_float4(ev.p.x, ev.p.y, ev.p.z, tr > 0.0f ? ev.sn[0] : 0.0f);
+        rec->med[rec->n] = medium;
+      }
+      rec->n++;
+    }
     T *= tr;
     ray.o = ev.p;
     ray.maxt -= ev.t;
@@ -103,7 +114,7 @@ __device__ inline float simple_ratio_tracking(const SceneView &S, int medium, Ra
 template <int F, bool GRAD>
 __device__ inline V3 simple_sample_emitter(const SceneView &S, int medium, V3 p, Pcg32 &rng, DirSample &ds,
                                            const LdsStack &stack, LocalCounters &cnt, float adj,
-                                           const GradTargets *G) {
+                                           const GradTargets *G, NeeRecord *rec = nullptr) {
   const float u1 = rng.next_1d();
   const float u2 = rng.next_1d();
   V3 ev = sample_emitter_direction(S, p, u1, u2, ds);
@@ -117,7 +128,7 @@ __device__ inline V3 simple_sample_emitter(const SceneView &S, int medium, V3 p,
   cnt.add(kCntTraversals, 1);
   const SurfInt si = make_si(S, r, h);
   if (!si.valid) return v3(0.f);
-  return ev * simple_ratio_tracking<F, GRAD>(S, medium, r, si.t, rng, cnt, adj, G);
+  return ev * simple_ratio_tracking<F, GRAD>(S, medium, r, si.t, rng, cnt, adj, G, rec);
 }
 
 // Where a path starts: from the camera (cross the medium boundary first,
@@ -307,7 +318,10 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
     if (use_nee && did_scatter && active) {
       const Pcg32 nee_rng = rng;
       DirSample ds;
-      const V3 em = simple_sample_emitter<F, false>(S, medium, mev.p, rng, ds, stack, cnt, 0.0f, G);
+      NeeRecord rec;
+      rec.n = 0;
+      const V3 em = simple_sample_emitter<F, false>(S, medium, mev.p, rng, ds, stack, cnt, 0.0f, G,
+                                                    ADJ ? &rec : nullptr);
       const float mis = mis_weight(ds.pdf, kInv4Pi);
       const V3 c = thr * em * (kInv4Pi * mis);
       if (ADJ) {
@@ -316,7 +330,21 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
         if (cs != 0.0f) {
           Pcg32 r2 = nee_rng;
           DirSample ds2;
-          (void)simple_sample_emitter<F, true>(S, medium, mev.p, r2, ds2, stack, cnt, cs, G);
+          if (rec.n <= kNeeRecords) {
+            // the replay's scatter-adds, from the recorded collisions
+            float *g = G->g[M.sigma_param];
+            for (int j = 0; g && j < rec.n; ++j) {
+              const float sn = rec.e[j].w;
+              if (sn != 0.0f) {
+                GridFoot f;
+                grid_footprint(S.params[M.sigma_param], M, v3(rec.e[j].x, rec.e[j].y, rec.e[j].z), f);
+                scatter_grid(g, f, -cs * M.scale / sn);
+                cnt.add(kCntGradAtomics, 8);
+              }
+            }
+          } else {
+            (void)simple_sample_emitter<F, true>(S, medium, mev.p, r2, ds2, stack, cnt, cs, G);
+          }
         }
       } else {
         result += c;
