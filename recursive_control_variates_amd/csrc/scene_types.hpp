@@ -101,7 +101,7 @@ struct SensorInfo {
 
 // Scene features a volume kernel is compiled for (its template parameter F):
 // code for an absent feature is left out, which keeps the register count of
-// the common configurations low enough for three waves per SIMD.
+// the common configurations low enough for four waves per SIMD.
 constexpr int kFeatSurfaces = 1;    // a shape whose BSDF is not null (diffuse shading, surface NEE)
 constexpr int kFeatAlbedoGrid = 2;  // a medium whose albedo is a grid
 constexpr int kFeatSupergrid = 4;   // a medium with a majorant supergrid
