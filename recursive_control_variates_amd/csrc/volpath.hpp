@@ -766,6 +766,7 @@ __device__ void volpath_adjoint_samples(const SceneView &S, const IntegArgs &I, 
                                         int px, int py, int64_t pixel, int spp, int s0, int s1, V3 dL,
                                         const GradTargets *G, const LdsStack &stack, LocalCounters &cnt) {
   VolPath<1, true, F> t;
+  t.L[0] = v3(0.f);  // read (unused) when the first sample starts
   bool live = false, want_replay = false;
   int s = s0;
   while (true) {
