@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kMaxBlock = 256;      // launch bound of the render kernels
 constexpr int kDefaultBlock = 256;  // threads per block when the descriptor says 0
+constexpr int kAdjUnitSamples = 16;  // camera samples per work unit of the persistent adjoint
 
 template <int NS, int INTEG, int F>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void render_kernel(
@@ -137,6 +138,68 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
     }
   }
 }
+
+// Path-replay adjoint of the volume path tracer as a persistent launch: one
+// grid of resident waves whose lanes pull work units (a pixel's chunk of
+// kAdjUnitSamples camera samples) from a device counter, a wavefront's
+// requests combined into one atomic, and keep tracing until the units run
+// out, so no wave waits at the end of the launch for a few long pixels. The
+// result per sample is that of adjoint_kernel; only the order of the gradient
+// scatter-adds differs.
+template <int F>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH>))) void
+adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2,
+                          const float *__restrict__ grad_img, GradTargets G, unsigned *queue, unsigned total_units) {
+  extern __shared__ int lds_stack[];  // kStackSize ints per thread
+  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  const SensorInfo &sen = S.sensors[sensor];
+  const int film_w = sen.film_w;
+  LocalCounters cnt{S.counters};
+  VolPath<1, true, F> t;
+  t.L[0] = v3(0.f);
+  bool live = false, want_replay = false;
+  int s = 0, s_end = 0, px = 0, py = 0;
+  int64_t pixel = 0;
+  V3 dL = v3(0.f);
+  const float inv = float(spp);
+  while (true) {
+    if (!live) {
+      if (!want_replay && s >= s_end) {
+        // the lanes that need a unit in this trip share one atomic
+        const uint64_t m = __ballot(1);
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned base = 0;
+        if (int(__lane_id()) == leader) base = atomicAdd(queue, unsigned(__popcll(m)));
+        base = __shfl(base, leader, 64);
+        const unsigned u = base + unsigned(__popcll(m & __lanemask_lt()));
+        if (u >= total_units) break;
+        pixel = int64_t(u >> chunk_log2);
+        const int chunk = int(u & ((1u << chunk_log2) - 1u));
+        px = int(pixel % film_w);
+        py = int(pixel / film_w);
+        s = sample_end(chunk - 1, spp, chunk_log2);
+        s_end = sample_end(chunk, spp, chunk_log2);
+        dL = v3(grad_img[3 * pixel] / inv, grad_img[3 * pixel + 1] / inv, grad_img[3 * pixel + 2] / inv);
+        if (!any_nonzero(dL)) {
+          s = s_end;
+          continue;
+        }
+      }
+      const AdjState a{dL, t.L[0]};
+      Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+      const Ray r = camera_ray(sen, px, py, rng);
+      t.begin(rng, r, cnt, want_replay);
+      if (want_replay) t.adj = a;
+    }
+    live = t.bounce(S, I, stack, cnt, &G);
+    if (!live) {
+      want_replay = !t.replay && any_nonzero(t.L[0]);
+      if (!want_replay) ++s;
+    }
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned *p) { p[threadIdx.x] = 0u; }
 
 __global__ void zero_counters_kernel(unsigned long long *c) {
   if (threadIdx.x < kNumCounters) c[threadIdx.x] = 0ull;
@@ -315,6 +378,23 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>), dim3(grid),
                          dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img,
                          G);
+    });
+  else if (d->type == RCV_INTEGRATOR_VOLPATH && !d->samples_per_thread)
+    with_features(s->features(), [&](auto f) {
+      // units of kAdjUnitSamples samples; a grid of the waves that fit
+      constexpr int F = decltype(f)::value;
+      int ucl2 = 0;
+      while ((1 << ucl2) < 64 && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
+      const unsigned units = unsigned(n_pix << ucl2);
+      int per_cu = 0, cus = 0, dev = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_persistent_kernel<F>, block,
+                                                         stack_bytes(block));
+      const unsigned pgrid = unsigned(std::max(1, per_cu) * std::max(1, cus));
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
+      hipLaunchKernelGGL(adjoint_persistent_kernel<F>, dim3(std::min(pgrid, grid)), dim3(block), stack_bytes(block),
+                         stream, s->view, I, sensor, seed, spp, ucl2, grad_img, G, s->work_queue, units);
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH)
     with_features(s->features(), [&](auto f) {

@@ -359,6 +359,8 @@ int Scene::upload(hipStream_t stream) {
   const size_t o_maj = put(maj_init.data(), maj_init.size() * 4);
   std::vector<unsigned long long> cnt(kNumCounters, 0ull);
   const size_t o_cnt = put(cnt.data(), cnt.size() * 8);
+  const unsigned q_init[4] = {0u, 0u, 0u, 0u};
+  const size_t o_q = put(q_init, sizeof(q_init));
 
   RCV_TRY_HIP(hipMalloc(&dev_block, blob.size()), "rcv_scene_commit: hipMalloc");
   RCV_TRY_HIP(hipMemcpyAsync(dev_block, blob.data(), blob.size(), hipMemcpyHostToDevice, stream),
@@ -385,6 +387,7 @@ int Scene::upload(hipStream_t stream) {
   majorant_keys = reinterpret_cast<uint32_t *>(base + o_maj);
   v.majorants = reinterpret_cast<const float *>(majorant_keys);
   counters = reinterpret_cast<unsigned long long *>(base + o_cnt);
+  work_queue = reinterpret_cast<unsigned *>(base + o_q);
   v.counters = nullptr;
   v.n_tris = n_tris;
   v.n_sph = n_sph;

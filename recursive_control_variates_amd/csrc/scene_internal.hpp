@@ -41,6 +41,7 @@ struct Scene {
   MediumInfo *media_dev = nullptr;  // device copy of `media`
   int batch_refs = 0;               // batches holding this scene (no relayout then)
   unsigned long long *counters = nullptr;
+  unsigned *work_queue = nullptr;  // unit counter of the persistent adjoint launches
   int n_nodes = 0;
   float bs_center[3] = {0, 0, 0};
   float bs_radius = 0;
