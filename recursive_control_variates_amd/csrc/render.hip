@@ -199,6 +199,96 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
   }
 }
 
+// The volume path tracer's render as a persistent launch (see
+// adjoint_persistent_kernel): a lane traces the samples of a work unit (a
+// pixel's chunk) in order and stores their sum in the unit's slot of
+// `partial`; reduce_units_kernel then forms each pixel's mean from its units
+// in order, so the image does not depend on which lane took which unit.
+template <int NS, int F>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH>))) void
+render_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2,
+                         float *__restrict__ partial, unsigned *queue, unsigned total_units) {
+  extern __shared__ int lds_stack[];  // kStackSize ints per thread
+  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  const SensorInfo &sen = S.sensors[sensor];
+  const int film_w = sen.film_w;
+  LocalCounters cnt{S.counters};
+  VolPath<NS, false, F> t;
+  bool live = false;
+  int s = 0, s_end = 0, px = 0, py = 0;
+  unsigned u = 0;
+  int64_t pixel = 0;
+  V3 acc[NS];
+  while (true) {
+    if (!live) {
+      if (s >= s_end) {
+        const uint64_t m = __ballot(1);
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned base = 0;
+        if (int(__lane_id()) == leader) base = atomicAdd(queue, unsigned(__popcll(m)));
+        base = __shfl(base, leader, 64);
+        u = base + unsigned(__popcll(m & __lanemask_lt()));
+        if (u >= total_units) break;
+        pixel = int64_t(u >> chunk_log2);
+        const int chunk = int(u & ((1u << chunk_log2) - 1u));
+        px = int(pixel % film_w);
+        py = int(pixel / film_w);
+        s = sample_end(chunk - 1, spp, chunk_log2);
+        s_end = sample_end(chunk, spp, chunk_log2);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = v3(0.f);
+      }
+      Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+      const Ray r = camera_ray(sen, px, py, rng);
+      t.begin(rng, r, cnt, false);
+    }
+    live = t.bounce(S, I, stack, cnt, nullptr);
+    if (!live) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) acc[k] += t.L[k];
+      if (++s >= s_end) {
+        float *o = partial + int64_t(u) * (3 * NS);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          o[3 * k] = acc[k].x;
+          o[3 * k + 1] = acc[k].y;
+          o[3 * k + 2] = acc[k].z;
+        }
+      }
+    }
+  }
+}
+
+// Pixel mean from the unit sums, units in order (a pixel of two units adds
+// them as the cross-lane reduction of render_kernel does).
+template <int NS>
+__global__ void reduce_units_kernel(const float *__restrict__ partial, int64_t n_pix, int chunk_log2, int spp,
+                                    float *__restrict__ img_old, float *__restrict__ img_new) {
+  const int64_t pixel = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (pixel >= n_pix) return;
+  const int units = 1 << chunk_log2;
+  const float inv = float(spp);
+  for (int k = 0; k < NS; ++k) {
+    float x = 0.f, y = 0.f, z = 0.f;
+    for (int c = 0; c < units; ++c) {
+      const float *src = partial + ((pixel << chunk_log2) + c) * (3 * NS) + 3 * k;
+      if (c == 0) {
+        x = src[0];
+        y = src[1];
+        z = src[2];
+      } else {
+        x += src[0];
+        y += src[1];
+        z += src[2];
+      }
+    }
+    float *o = (NS == 2 ? (k == 0 ? img_old : img_new) : img_new) + 3 * pixel;
+    o[0] = x / inv;
+    o[1] = y / inv;
+    o[2] = z / inv;
+  }
+}
+
 __global__ void zero_u32_kernel(unsigned *p) { p[threadIdx.x] = 0u; }
 
 __global__ void zero_counters_kernel(unsigned long long *c) {
@@ -328,6 +418,33 @@ int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint
     with_volume_features(s->features(), [&](auto f) {
       launch_render_kernel<1, RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>(L);
     });
+  } else if (d->type == RCV_INTEGRATOR_VOLPATH && !d->samples_per_thread) {
+    // persistent: units of >= kAdjUnitSamples samples, at most
+    // kRenderMaxUnitsPerPixel per pixel
+    int ucl2 = 0;
+    while ((1 << (ucl2 + 1)) <= kRenderMaxUnitsPerPixel && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
+    const unsigned units = unsigned(n_pix << ucl2);
+    RCV_REQUIRE(int64_t(units) * 3 * ns <= s->partial_cap, "%s: unit buffer too small", fn);
+    int cus = 0, dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
+    auto go = [&](auto kern) {
+      int per_cu = 0;
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, stack_bytes(block));
+      const unsigned pgrid = std::min(unsigned(std::max(1, per_cu) * std::max(1, cus)), grid);
+      hipLaunchKernelGGL(kern, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp,
+                         ucl2, s->partial, s->work_queue, units);
+    };
+    if (ns == 2) {
+      with_features(s->features(), [&](auto f) { go(render_persistent_kernel<2, decltype(f)::value>); });
+      hipLaunchKernelGGL(reduce_units_kernel<2>, dim3(ceil_div(n_pix, 256)), dim3(256), 0, stream, s->partial, n_pix,
+                         ucl2, spp, img_old, img_new);
+    } else {
+      with_features(s->features(), [&](auto f) { go(render_persistent_kernel<1, decltype(f)::value>); });
+      hipLaunchKernelGGL(reduce_units_kernel<1>, dim3(ceil_div(n_pix, 256)), dim3(256), 0, stream, s->partial, n_pix,
+                         ucl2, spp, img_old, img_new);
+    }
   } else if (d->type == RCV_INTEGRATOR_VOLPATH) {
     if (ns == 2)
       with_features(s->features(), [&](auto f) { launch_render_kernel<2, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>(L); });

@@ -363,6 +363,15 @@ int Scene::upload(hipStream_t stream) {
   const size_t o_q = put(q_init, sizeof(q_init));
 
   RCV_TRY_HIP(hipMalloc(&dev_block, blob.size()), "rcv_scene_commit: hipMalloc");
+  // unit sums of persistent renders: up to kRenderMaxUnitsPerPixel units of
+  // (old, new) RGB per pixel of the largest sensor
+  {
+    int64_t max_pix = 0;
+    for (auto &sen : sensors) max_pix = std::max(max_pix, int64_t(sen.film_w) * sen.film_h);
+    partial_cap = max_pix * kRenderMaxUnitsPerPixel * 6;
+    if (partial_cap > 0)
+      RCV_TRY_HIP(hipMalloc(&partial, size_t(partial_cap) * sizeof(float)), "rcv_scene_commit: hipMalloc");
+  }
   RCV_TRY_HIP(hipMemcpyAsync(dev_block, blob.data(), blob.size(), hipMemcpyHostToDevice, stream),
               "rcv_scene_commit: upload");
   RCV_TRY_HIP(hipStreamSynchronize(stream), "rcv_scene_commit: synchronize");
@@ -514,6 +523,7 @@ int Scene::features() const {
 Scene::~Scene() {
   if (dev_block) (void)hipFree(dev_block);
   if (sg_block) (void)hipFree(sg_block);
+  if (partial) (void)hipFree(partial);
 }
 
 }  // namespace rcv

@@ -9,6 +9,10 @@
 
 namespace rcv {
 
+// Persistent volume renders split a pixel into at most this many work units;
+// their unit-sum buffer is sized for it at scene commit.
+constexpr int kRenderMaxUnitsPerPixel = 2;
+
 struct Param {
   int kind = 0;
   int nx = 0, ny = 0, nz = 0, ch = 0;
@@ -41,7 +45,9 @@ struct Scene {
   MediumInfo *media_dev = nullptr;  // device copy of `media`
   int batch_refs = 0;               // batches holding this scene (no relayout then)
   unsigned long long *counters = nullptr;
-  unsigned *work_queue = nullptr;  // unit counter of the persistent adjoint launches
+  unsigned *work_queue = nullptr;  // unit counter of the persistent launches
+  float *partial = nullptr;        // per work unit radiance sums of a persistent render
+  int64_t partial_cap = 0;         // floats
   int n_nodes = 0;
   float bs_center[3] = {0, 0, 0};
   float bs_radius = 0;
