@@ -13,6 +13,9 @@
 // strided by the block size).
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "common.hpp"
 #include "path.hpp"
@@ -300,6 +303,34 @@ __global__ void zero_counters_kernel(unsigned long long *c) {
 
 static size_t stack_bytes(int block) { return size_t(kStackSize) * block * sizeof(int); }
 
+// Blocks of `kern` the current device holds at once (compute units x
+// resident blocks per unit), cached per (device, kernel, block size).
+static unsigned resident_grid(const void *kern, int block) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void *, int>, unsigned> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_tuple(dev, kern, block);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, per_cu = 0;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, stack_bytes(block));
+  const unsigned g = unsigned(std::max(1, per_cu) * std::max(1, cus));
+  cache[key] = g;
+  return g;
+}
+
+// Work-unit split of a persistent volume launch: the chunk count of the
+// static launch (enough units to fill the device), raised while a unit keeps
+// at least kAdjUnitSamples samples.
+static int unit_chunk_log2(int spp, int static_cl2) {
+  int ucl2 = 0;
+  while ((1 << ucl2) < 64 && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
+  return std::max(ucl2, static_cl2);
+}
+
 int plan_launch(const rcv_integrator_desc *d, int64_t n_pix, int spp, int &spt, int &chunk_log2, int max_spt,
                 int64_t min_threads) {
   if (d && d->block_size != 0 && (d->block_size < 64 || d->block_size > kMaxBlock || d->block_size % 64 != 0))
@@ -418,21 +449,17 @@ int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint
     with_volume_features(s->features(), [&](auto f) {
       launch_render_kernel<1, RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>(L);
     });
-  } else if (d->type == RCV_INTEGRATOR_VOLPATH && !d->samples_per_thread) {
-    // persistent: units of >= kAdjUnitSamples samples, at most
-    // kRenderMaxUnitsPerPixel per pixel
-    int ucl2 = 0;
-    while ((1 << (ucl2 + 1)) <= kRenderMaxUnitsPerPixel && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
+  } else if (d->type == RCV_INTEGRATOR_VOLPATH && !d->samples_per_thread && (1 << cl2) <= kRenderMaxUnitsPerPixel) {
+    // persistent, with at most kRenderMaxUnitsPerPixel units per pixel (the
+    // unit-sum buffer); smaller images that need more units per pixel to
+    // fill the device take the static launch below
+    int ucl2 = cl2;
+    while ((2 << ucl2) <= kRenderMaxUnitsPerPixel && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
     const unsigned units = unsigned(n_pix << ucl2);
     RCV_REQUIRE(int64_t(units) * 3 * ns <= s->partial_cap, "%s: unit buffer too small", fn);
-    int cus = 0, dev = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
     auto go = [&](auto kern) {
-      int per_cu = 0;
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, stack_bytes(block));
-      const unsigned pgrid = std::min(unsigned(std::max(1, per_cu) * std::max(1, cus)), grid);
+      const unsigned pgrid = std::min(resident_grid(reinterpret_cast<const void *>(kern), block), grid);
       hipLaunchKernelGGL(kern, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp,
                          ucl2, s->partial, s->work_queue, units);
     };
@@ -498,20 +525,17 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH && !d->samples_per_thread)
     with_features(s->features(), [&](auto f) {
-      // units of kAdjUnitSamples samples; a grid of the waves that fit
+      // the waves that fit, pulling units of >= kAdjUnitSamples samples
+      // (fewer where the image is too small to fill the device otherwise)
       constexpr int F = decltype(f)::value;
-      int ucl2 = 0;
-      while ((1 << ucl2) < 64 && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
+      const int ucl2 = unit_chunk_log2(spp, cl2);
       const unsigned units = unsigned(n_pix << ucl2);
-      int per_cu = 0, cus = 0, dev = 0;
-      (void)hipGetDevice(&dev);
-      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_persistent_kernel<F>, block,
-                                                         stack_bytes(block));
-      const unsigned pgrid = unsigned(std::max(1, per_cu) * std::max(1, cus));
+      const unsigned pgrid =
+          std::min(resident_grid(reinterpret_cast<const void *>(adjoint_persistent_kernel<F>), block),
+                   ceil_div(int64_t(units), block));
       hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
-      hipLaunchKernelGGL(adjoint_persistent_kernel<F>, dim3(std::min(pgrid, grid)), dim3(block), stack_bytes(block),
-                         stream, s->view, I, sensor, seed, spp, ucl2, grad_img, G, s->work_queue, units);
+      hipLaunchKernelGGL(adjoint_persistent_kernel<F>, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view,
+                         I, sensor, seed, spp, ucl2, grad_img, G, s->work_queue, units);
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH)
     with_features(s->features(), [&](auto f) {
