@@ -218,11 +218,11 @@ def kernel_roofline(args, st):
     sc.set_counting(False)
     a_bytes = 32 * (ac["collisions"] + ac["nee_steps"]) + 4 * ac["grad_atomics"] + 12 * n_pix
     out = {}
-    akern = ("adjoint_kernel<VOLPATH> (path replay)" if adj is integ
+    akern = ("adjoint_persistent_kernel<VOLPATH> (path replay)" if adj is integ
              else "adjoint_kernel<VOLPATH_SIMPLE> (differential ratio tracking)")
     for name, ms, by, cnt, kern in (("volume_adjoint", a_ms, a_bytes, ac, akern),
                                     ("volume_twostate", r_ms, r_bytes, rc,
-                                     "render_kernel<2, VOLPATH> (two-state delta tracking)")):
+                                     "render_persistent_kernel<2, VOLPATH> (two-state delta tracking)")):
         ach = by / (ms * 1e-3) / 1e9
         out[name] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": _traffic(name), "kernel": kern,
