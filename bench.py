@@ -316,7 +316,8 @@ def cpu_baseline(args):
     d = scene_desc.describe(scenes.volume_scene_dict(init, film_res=args.res, spp=args.spp))
     threads = min(16, os.cpu_count() or 1)
     n_pix_total = d.sensors[0].film_width * d.sensors[0].film_height
-    # middle band of the film (rays through the medium), growing until the budget is used
+    # middle band of the film (rays through the medium), growing until the budget is used;
+    # once the band is the whole film, further whole-film passes (new seeds) are added
     n = 256
     spent = 0.0
     while True:
@@ -329,7 +330,16 @@ def cpu_baseline(args):
         if dt > args.cpu_seconds / 3 or spent > args.cpu_seconds or n >= n_pix_total:
             break
         n = min(n_pix_total, n * 4)
-    msps = n * args.spp / dt / 1e6
+    passes = 1
+    while n >= n_pix_total and dt < args.cpu_seconds / 3 and spent < args.cpu_seconds:
+        t0 = time.perf_counter()
+        R.render(d, 1, args.max_depth, args.max_depth, 7 + passes, args.spp, params_old=[target],
+                 params_new=[init], precision=32, n_threads=threads, pixel_range=(0, n))
+        step = time.perf_counter() - t0
+        dt += step
+        spent += step
+        passes += 1
+    msps = passes * n * args.spp / dt / 1e6
     cpu = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -340,7 +350,8 @@ def cpu_baseline(args):
     except OSError:
         pass
     return {"value": round(msps, 4), "unit": "Msamples/s (two-state primal paths)", "cores": threads,
-            "kind": "port", "sample": f"{n} pixels x {args.spp} spp of the 4x256^2 film (central band), "
+            "kind": "port", "sample": f"{passes} pass(es) of {n} pixels x {args.spp} spp of the 4x256^2 film "
+                                      f"({'whole film' if n >= n_pix_total else 'central band'}), "
                                       f"two-state primal, float32, {dt:.2f}s on {cpu}"}
 
 
@@ -466,6 +477,15 @@ def bench_diff(args):
         integ.adjoint(sc, p, gimg, 0, 2, spp)
         torch.cuda.synchronize()
         out[f"adjoint_{name}_ms"] = round(_event_time(lambda r: integ.adjoint(sc, p, gimg, 0, 20 + r, spp), 3), 3)
+    # the same launches with the k = 8 majorant supergrid (16^3 cells of local majorants)
+    sc.set_majorant_factor(0, 8)
+    ts.render_twostates(sc, 0, 1, spp)
+    prb.adjoint(sc, p, gimg, 0, 2, spp)
+    torch.cuda.synchronize()
+    out["supergrid_8"] = {
+        "twostate_ms": round(_event_time(lambda r: ts.render_twostates(sc, 0, 10 + r, spp), 3), 3),
+        "adjoint_prb_ms": round(_event_time(lambda r: prb.adjoint(sc, p, gimg, 0, 20 + r, spp), 3), 3)}
+    sc.set_majorant_factor(0, 0)
     log(f"[bench] diff timings {out}")
     # finite differences along directions on 32 seed-chosen voxels. The
     # two-state render of (theta + eps delta, theta - eps delta) and the
