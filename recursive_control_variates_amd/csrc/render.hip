@@ -188,15 +188,13 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
           continue;
         }
       }
-      const AdjState a{dL, t.L[0]};
       Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
       const Ray r = camera_ray(sen, px, py, rng);
-      t.begin(rng, r, cnt, want_replay);
-      if (want_replay) t.adj = a;
+      t.begin_adjoint(rng, r, cnt, want_replay, dL, t.L[0]);
     }
     live = t.bounce(S, I, stack, cnt, &G);
     if (!live) {
-      want_replay = !t.replay && any_nonzero(t.L[0]);
+      want_replay = !t.end_adjoint(S, cnt, &G);
       if (!want_replay) ++s;
     }
   }

@@ -17,6 +17,8 @@
 // (:171-173) does not change any result; its random number is still drawn.
 #pragma once
 
+#include <type_traits>
+
 #include "scene_device.hpp"
 
 namespace rcv {
@@ -400,14 +402,40 @@ struct AdjState {
   V3 Lrem;  // radiance still to be gathered along the replayed path
 };
 
+// The collisions of one primal pass of the adjoint: per collision the point
+// and d log(weight)/d(density) per unit of remaining radiance (scale /
+// sigma_t for a real collision, -scale / sigma_n for a null one), dLoss/dL .
+// L gathered before the collision, and the medium. Once the path's radiance
+// L is known, the collision's gradient is that coefficient times dLoss/dL .
+// (L - the gathered part), which is the replay pass's dLoss/dL . Lrem at the
+// same collision, so the path need not be traced a second time. A path with
+// more than kTapeRecords collisions is replayed instead.
+constexpr int kTapeRecords = 32;
+struct PathTape {
+  int n;
+  float4 e[kTapeRecords];  // point, coefficient
+  float s[kTapeRecords];   // dLoss/dL . L gathered before the collision
+  int med[kTapeRecords];
+};
+struct NoTape {};
+
+// One-pass (taped) adjoint for kernels whose only medium gradient is the
+// density's; the albedo grid's gradient needs the colour of the remaining
+// radiance, so kernels with an albedo grid keep the replay pass.
+template <int F>
+constexpr bool kTapeAdjoint = !(F & kFeatAlbedoGrid);
+
 // One camera sample of the two-state volumetric path tracer, advanced one
 // bounce per call (the body of the reference's loop, :175-380), so that a
 // kernel can start a lane's next camera sample as soon as its path ends
 // instead of waiting for the longest path of its wavefront. NS = 2: L[0] =
-// old (H), L[1] = new (F). ADJ (NS = 1) enables the replay pass of path-replay
-// backpropagation (replay = true): with the primal radiance in adj.Lrem, the
-// same random numbers are replayed and d(log weight)/d(theta) at each
-// collision times the radiance still to come is scattered into the gradients.
+// old (H), L[1] = new (F). ADJ (NS = 1) is the adjoint of path-replay
+// backpropagation: d(log weight)/d(theta) at each collision times the
+// radiance still to come is scattered into the gradients. With a tape
+// (kTapeAdjoint) the primal pass records the collisions and scatters the NEE
+// transmittance gradients at once, and end_adjoint scatters the collisions';
+// otherwise the replay pass (replay = true, the primal radiance in adj.Lrem)
+// replays the same random numbers and scatters as it goes.
 template <int NS, bool ADJ, int F>
 struct VolPath {
   Pcg32 rng;
@@ -417,8 +445,9 @@ struct VolPath {
   V3 last_p;
   float last_pdf;
   int depth, medium;
-  bool needs_int, specular_chain, replay;
+  bool needs_int, specular_chain, replay, nee_grad;
   AdjState adj;
+  std::conditional_t<ADJ, PathTape, NoTape> tape;
 
   __device__ void begin(const Pcg32 &r, const Ray &camera, LocalCounters &cnt, bool is_replay) {
     rng = r;
@@ -441,8 +470,51 @@ struct VolPath {
     last_pdf = 1.0f;
     medium = -1;
     replay = ADJ && is_replay;
+    nee_grad = false;
     (void)rng.next_1d();  // colour channel for free-flight sampling (:171-173)
     cnt.add(kCntPaths, 1);
+  }
+
+  // Adjoint: start the primal pass (is_replay false) or the replay pass of a
+  // camera sample with dLoss/dL = dL and primal radiance Lp (read by the
+  // replay only). NEE transmittance gradients do not depend on the radiance
+  // still to come: a taped adjoint scatters them in the primal pass, an
+  // untaped one in the replay.
+  __device__ void begin_adjoint(const Pcg32 &r, const Ray &camera, LocalCounters &cnt, bool is_replay, V3 dL,
+                                V3 Lp) {
+    begin(r, camera, cnt, is_replay);
+    if constexpr (ADJ) {
+      adj.dL = dL;
+      adj.Lrem = Lp;
+      nee_grad = kTapeAdjoint<F> ? !is_replay : is_replay;
+      tape.n = 0;
+    }
+  }
+
+  // Adjoint, once bounce() has returned false: true when the camera sample is
+  // done (its taped collisions scattered), false when it needs its replay
+  // pass (an untaped kernel, or a tape overflow).
+  __device__ bool end_adjoint(const SceneView &S, LocalCounters &cnt, const GradTargets *G) {
+    if constexpr (ADJ) {
+      if (replay || !any_nonzero(L[0])) return true;
+      if (kTapeAdjoint<F> && tape.n <= kTapeRecords) {
+        const float total = dot(adj.dL, L[0]);
+        for (int j = 0; j < tape.n; ++j) {
+          const float lr = total - tape.s[j];
+          if (lr != 0.0f) {
+            const MediumInfo &M = S.media[tape.med[j]];
+            const float4 e = tape.e[j];
+            GridFoot f;
+            grid_footprint(S.params[M.sigma_param], M, v3(e.x, e.y, e.z), f);
+            scatter_grid(G->g[M.sigma_param], f, lr * e.w);
+            cnt.add(kCntGradAtomics, 8);
+          }
+        }
+        return true;
+      }
+      return false;
+    }
+    return true;
   }
 
   // One bounce; false once the path has ended.
@@ -537,6 +609,21 @@ struct VolPath {
   #pragma unroll
       for (int k = 0; k < NS; ++k) thr[k] *= w[k];
     }
+    if constexpr (ADJ) {
+      if (!replay && kTapeAdjoint<F> && (act_null || act_scatter)) {
+        // record the collision (its gradient needs the path's radiance)
+        const MediumInfo &M = S.media[medium];
+        if (G->g[M.sigma_param] && !(act_null && mev.sn[0] == 0.0f)) {
+          if (tape.n < kTapeRecords) {
+            const float coef = act_null ? -M.scale / mev.sn[0] : M.scale / mev.st[0];
+            tape.e[tape.n] = make_float4(mev.p.x, mev.p.y, mev.p.z, coef);
+            tape.s[tape.n] = dot(adj.dL, L[0]);
+            tape.med[tape.n] = medium;
+          }
+          tape.n++;
+        }
+      }
+    }
     if (ADJ && replay && (act_null || act_scatter)) {
       // d log(weight) / d theta at this collision times the radiance that
       // the weight multiplies (everything gathered after this point).
@@ -627,7 +714,7 @@ struct VolPath {
       NeeRecord rec;
       rec.n = 0;
       nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G,
-                                  ADJ && replay ? &rec : nullptr);
+                                  ADJ && nee_grad ? &rec : nullptr);
       V3 nw[NS];
       float nee_pdf;
       if (active_e_surface) {
@@ -648,7 +735,7 @@ struct VolPath {
         c[k] = thr[k] * nw[k] * mis_em * em[k];
         L[k] += c[k];
       }
-      if (ADJ && replay) {
+      if (ADJ && nee_grad) {
         const float cs = dot(adj.dL, c[0]);
         if (cs != 0.0f && any_nonzero(c[0])) {
           if (rec.n <= kNeeRecords) {
@@ -673,8 +760,8 @@ struct VolPath {
                                          G);
           }
         }
-        adj.Lrem -= c[0];
       }
+      if (ADJ && replay) adj.Lrem -= c[0];
     }
 
     // ---- BSDF sampling (:347-380)
@@ -758,9 +845,11 @@ __device__ void volpath_render_samples(const SceneView &S, const IntegArgs &I, c
 }
 
 // Path-replay adjoint of the camera samples [s0, s1) of one pixel (NEW
-// state): per sample a primal pass for its radiance L, then, when L is not
-// zero, the replay pass with the same random numbers. dimage/dL = 1/spp (box
-// filter) is folded into dL.
+// state): per sample a primal pass for its radiance L that records its
+// collisions (scattered once L is known), or, for untaped kernels and tape
+// overflows, the primal pass and then, when L is not zero, the replay pass
+// with the same random numbers. dimage/dL = 1/spp (box filter) is folded
+// into dL.
 template <int F>
 __device__ void volpath_adjoint_samples(const SceneView &S, const IntegArgs &I, const SensorInfo &sen, uint32_t seed,
                                         int px, int py, int64_t pixel, int spp, int s0, int s1, V3 dL,
@@ -772,15 +861,13 @@ __device__ void volpath_adjoint_samples(const SceneView &S, const IntegArgs &I, 
   while (true) {
     if (!live) {
       if (s >= s1) break;
-      const AdjState a{dL, t.L[0]};
       Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
       const Ray r = camera_ray(sen, px, py, rng);
-      t.begin(rng, r, cnt, want_replay);
-      if (want_replay) t.adj = a;
+      t.begin_adjoint(rng, r, cnt, want_replay, dL, t.L[0]);
     }
     live = t.bounce(S, I, stack, cnt, G);
     if (!live) {
-      want_replay = !t.replay && any_nonzero(t.L[0]);
+      want_replay = !t.end_adjoint(S, cnt, G);
       if (!want_replay) ++s;
     }
   }
