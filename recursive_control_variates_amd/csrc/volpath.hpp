@@ -419,11 +419,18 @@ struct PathTape {
 };
 struct NoTape {};
 
-// One-pass (taped) adjoint for kernels whose only medium gradient is the
-// density's; the albedo grid's gradient needs the colour of the remaining
-// radiance, so kernels with an albedo grid keep the replay pass.
+// One-pass (taped) adjoint: for kernels whose media have a majorant
+// supergrid and whose only medium gradient is the density's. Records live in
+// per-lane scratch, which does not fit in cache, so a record written and
+// read back from HBM costs more than tracking the collision again (an
+// L2-resident grid lookup) when paths collide often, as under one global
+// majorant in an optically thick medium; with a supergrid, tracking a
+// collision again costs a DDA walk and paths collide less. The albedo grid's
+// gradient needs the colour of the remaining radiance, so kernels with an
+// albedo grid keep the replay pass. The mode is uniform per kernel: lanes of
+// one wavefront in different modes execute both modes' work.
 template <int F>
-constexpr bool kTapeAdjoint = !(F & kFeatAlbedoGrid);
+constexpr bool kTapeAdjoint = (F & kFeatSupergrid) && !(F & kFeatAlbedoGrid);
 
 // One camera sample of the two-state volumetric path tracer, advanced one
 // bounce per call (the body of the reference's loop, :175-380), so that a

@@ -48,6 +48,32 @@ def test_adjoint_matches_oracle_same_samples(cuda, factor):
     assert abs(g_gpu.sum() - g_cpu.sum()) < 0.03 * np.abs(g_cpu).sum()
 
 
+def test_adjoint_with_tape_overflows_matches_oracle_same_samples(cuda):
+    # With a supergrid the adjoint records each path's collisions in its
+    # primal pass and scatters them once the radiance is known; a path with
+    # more collisions than the record holds is replayed (collision terms only,
+    # its NEE terms were scattered in the primal pass). A thick, bright medium
+    # with deep paths makes both kinds frequent.
+    announce("adjoint vs oracle fp32 (supergrid, thick, tape overflows)")
+    rng = np.random.default_rng(4)
+    grid = (0.2 + rng.uniform(0, 0.8, (5, 5, 5, 1))).astype(np.float32)
+    w, h, spp, depth = 8, 8, 16, 40
+    sc = rcv.load_dict(volume_box_scene(grid, albedo=0.95, scale=40.0, w=w, h=h, spp=spp, majorant_factor=2))
+    gimg = rng.uniform(0.5, 1.5, (h, w, 3))
+    integ = rcv.load_dict({"type": "prbvolpath", "max_depth": depth, "rr_depth": depth})
+    sc.set_counting(True)
+    g_gpu = _adjoint(sc, integ, gimg, 13, spp)
+    cnt = sc.read_counters()
+    sc.set_counting(False)
+    replays = cnt["paths"] - w * h * spp
+    assert 0 < replays < w * h * spp, cnt
+    g_cpu = R.adjoint(sc.desc, depth, depth, 13, spp, gimg, precision=32)
+    scale = np.abs(g_cpu).max()
+    close = np.abs(g_gpu - g_cpu) <= 1e-3 * scale + 1e-3 * np.abs(g_cpu)
+    assert close.mean() > 0.97, close.mean()
+    assert abs(g_gpu.sum() - g_cpu.sum()) < 0.03 * np.abs(g_cpu).sum()
+
+
 def test_adjoint_matches_finite_differences(cuda):
     announce("adjoint vs FD")
     rng, grid, sc = _setup(2)
