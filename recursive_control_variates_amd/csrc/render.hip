@@ -200,6 +200,63 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
   }
 }
 
+// volpathsimple's adjoint as a persistent launch (see
+// adjoint_persistent_kernel): lanes pull work units (a pixel's chunk of
+// camera samples) from a device counter and run each sample's primal pass and
+// its adjoint with the same stream plus the gradient-only stream, so long
+// pixels do not hold their wavefronts at the end of the launch. The result
+// per sample is that of adjoint_kernel<VOLPATH_SIMPLE>; only the order of the
+// gradient scatter-adds differs.
+template <int F>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((
+    amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH_SIMPLE>))) void adjoint_simple_persistent_kernel(
+    SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2, const float *__restrict__ grad_img,
+    GradTargets G, unsigned *queue, unsigned total_units) {
+  extern __shared__ int lds_stack[];  // kStackSize ints per thread
+  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  const SensorInfo &sen = S.sensors[sensor];
+  const int film_w = sen.film_w;
+  LocalCounters cnt{S.counters};
+  const uint32_t alt_seed = simple_alt_seed(seed);
+  int s = 0, s_end = 0, px = 0, py = 0;
+  int64_t pixel = 0;
+  V3 dL = v3(0.f);
+  const float inv = float(spp);
+  while (true) {
+    if (s >= s_end) {
+      // the lanes that need a unit in this trip share one atomic
+      const uint64_t m = __ballot(1);
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      unsigned base = 0;
+      if (int(__lane_id()) == leader) base = atomicAdd(queue, unsigned(__popcll(m)));
+      base = __shfl(base, leader, 64);
+      const unsigned u = base + unsigned(__popcll(m & __lanemask_lt()));
+      if (u >= total_units) break;
+      pixel = int64_t(u >> chunk_log2);
+      const int chunk = int(u & ((1u << chunk_log2) - 1u));
+      px = int(pixel % film_w);
+      py = int(pixel / film_w);
+      s = sample_end(chunk - 1, spp, chunk_log2);
+      s_end = sample_end(chunk, spp, chunk_log2);
+      dL = v3(grad_img[3 * pixel] / inv, grad_img[3 * pixel + 1] / inv, grad_img[3 * pixel + 2] / inv);
+      if (!any_nonzero(dL)) {
+        s = s_end;
+        continue;
+      }
+    }
+    Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+    const Ray ray = camera_ray(sen, px, py, rng);
+    const Pcg32 rng0 = rng;
+    const V3 L =
+        simple_sample<false, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, nullptr, nullptr, nullptr);
+    AdjState adj{dL, L};
+    rng = rng0;
+    Pcg32 alt = lane_rng(alt_seed, uint32_t(pixel * spp + s));
+    (void)simple_sample<true, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, &adj, &alt, &G);
+    ++s;
+  }
+}
+
 // The volume path tracer's render as a persistent launch (see
 // adjoint_persistent_kernel): a lane traces the samples of a work unit (a
 // pixel's chunk) in order and stores their sum in the unit's slot of
@@ -515,7 +572,20 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   const int block = d->block_size ? d->block_size : kDefaultBlock;
   const unsigned grid = ceil_div(n_pix << cl2, block);
   if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
-  if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE)
+  if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE && !d->samples_per_thread)
+    with_volume_features(s->features(), [&](auto f) {
+      // as the volume path tracer's persistent adjoint below
+      constexpr int F = decltype(f)::value;
+      const int ucl2 = unit_chunk_log2(spp, cl2);
+      const unsigned units = unsigned(n_pix << ucl2);
+      const unsigned pgrid =
+          std::min(resident_grid(reinterpret_cast<const void *>(adjoint_simple_persistent_kernel<F>), block),
+                   ceil_div(int64_t(units), block));
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
+      hipLaunchKernelGGL(adjoint_simple_persistent_kernel<F>, dim3(pgrid), dim3(block), stack_bytes(block), stream,
+                         s->view, I, sensor, seed, spp, ucl2, grad_img, G, s->work_queue, units);
+    });
+  else if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE)
     with_volume_features(s->features(), [&](auto f) {
       hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>), dim3(grid),
                          dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img,
