@@ -454,7 +454,7 @@ struct VolPath {
   int depth, medium;
   bool needs_int, specular_chain, replay, nee_grad;
   AdjState adj;
-  std::conditional_t<ADJ, PathTape, NoTape> tape;
+  std::conditional_t<ADJ && kTapeAdjoint<F>, PathTape, NoTape> tape;
 
   __device__ void begin(const Pcg32 &r, const Ray &camera, LocalCounters &cnt, bool is_replay) {
     rng = r;
@@ -494,7 +494,7 @@ struct VolPath {
       adj.dL = dL;
       adj.Lrem = Lp;
       nee_grad = kTapeAdjoint<F> ? !is_replay : is_replay;
-      tape.n = 0;
+      if constexpr (kTapeAdjoint<F>) tape.n = 0;
     }
   }
 
@@ -502,9 +502,9 @@ struct VolPath {
   // done (its taped collisions scattered), false when it needs its replay
   // pass (an untaped kernel, or a tape overflow).
   __device__ bool end_adjoint(const SceneView &S, LocalCounters &cnt, const GradTargets *G) {
-    if constexpr (ADJ) {
+    if constexpr (ADJ && kTapeAdjoint<F>) {
       if (replay || !any_nonzero(L[0])) return true;
-      if (kTapeAdjoint<F> && tape.n <= kTapeRecords) {
+      if (tape.n <= kTapeRecords) {
         const float total = dot(adj.dL, L[0]);
         for (int j = 0; j < tape.n; ++j) {
           const float lr = total - tape.s[j];
@@ -520,6 +520,8 @@ struct VolPath {
         return true;
       }
       return false;
+    } else if constexpr (ADJ) {
+      return replay || !any_nonzero(L[0]);
     }
     return true;
   }
@@ -616,8 +618,8 @@ struct VolPath {
   #pragma unroll
       for (int k = 0; k < NS; ++k) thr[k] *= w[k];
     }
-    if constexpr (ADJ) {
-      if (!replay && kTapeAdjoint<F> && (act_null || act_scatter)) {
+    if constexpr (ADJ && kTapeAdjoint<F>) {
+      if (!replay && (act_null || act_scatter)) {
         // record the collision (its gradient needs the path's radiance)
         const MediumInfo &M = S.media[medium];
         if (G->g[M.sigma_param] && !(act_null && mev.sn[0] == 0.0f)) {
