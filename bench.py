@@ -219,7 +219,7 @@ def kernel_roofline(args, st):
     a_bytes = 32 * (ac["collisions"] + ac["nee_steps"]) + 4 * ac["grad_atomics"] + 12 * n_pix
     out = {}
     akern = ("adjoint_persistent_kernel<VOLPATH> (path replay)" if adj is integ
-             else "adjoint_simple_persistent_kernel<VOLPATH_SIMPLE> (differential ratio tracking)")
+             else "adjoint_sample_persistent_kernel<VOLPATH_SIMPLE> (differential ratio tracking)")
     for name, ms, by, cnt, kern in (("volume_adjoint", a_ms, a_bytes, ac, akern),
                                     ("volume_twostate", r_ms, r_bytes, rc,
                                      "render_persistent_kernel<2, VOLPATH> (two-state delta tracking)")):

@@ -89,6 +89,36 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
   }
 }
 
+// The adjoint of camera sample s of a pixel for the per-sample integrators
+// (surface path tracer, volpathsimple): the primal pass, then the adjoint pass
+// with the same random numbers -- a replay that scatters d(log weight)/d(theta)
+// times the radiance still to come (path tracer, skipped when L = 0), or
+// volpathsimple's adjoint with its gradient-only stream as well (DRT
+// contributes even where L = 0).
+template <int INTEG, int F>
+__device__ inline void adjoint_one_sample(const SceneView &S, const IntegArgs &I, const SensorInfo &sen,
+                                          uint32_t seed, int px, int py, int64_t pixel, int spp, int s, V3 dL,
+                                          const GradTargets &G, const LdsStack &stack, LocalCounters &cnt) {
+  Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+  const Ray ray = camera_ray(sen, px, py, rng);
+  const Pcg32 rng0 = rng;
+  V3 L[1];
+  if constexpr (INTEG == RCV_INTEGRATOR_VOLPATH_SIMPLE) {
+    L[0] = simple_sample<false, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, nullptr, nullptr, nullptr);
+    AdjState adj{dL, L[0]};
+    rng = rng0;
+    Pcg32 alt = lane_rng(simple_alt_seed(seed), uint32_t(pixel * spp + s));
+    (void)simple_sample<true, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, &adj, &alt, &G);
+  } else {
+    path_sample<1, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+    if (!any_nonzero(L[0])) return;
+    AdjState adj{dL, L[0]};
+    rng = rng0;
+    V3 L2[1];
+    path_sample<1, true>(S, I, rng, ray, L2, stack, cnt, &adj, &G);
+  }
+}
+
 // Path-replay adjoint (NEW state): primal pass for L, then a replay with the
 // same random numbers that scatters d(log weight)/d(theta) times the radiance
 // still to come into the parameter gradients. dimage/dL = 1/spp (box filter).
@@ -115,29 +145,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
                                  sample_end(chunk, spp, chunk_log2), dL, &G, stack, cnt);
     } else if (any_nonzero(dL)) {
       const int s_end = sample_end(chunk, spp, chunk_log2);
-      for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s) {
-        Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
-        const Ray ray = camera_ray(sen, px, py, rng);
-        const Pcg32 rng0 = rng;
-        V3 L[1];
-        if (INTEG == RCV_INTEGRATOR_VOLPATH_SIMPLE) {
-          // primal, then the adjoint with the same stream plus the
-          // gradient-only stream (DRT contributes even where L = 0)
-          L[0] = simple_sample<false, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, nullptr, nullptr,
-                                         nullptr);
-          AdjState adj{dL, L[0]};
-          rng = rng0;
-          Pcg32 alt = lane_rng(simple_alt_seed(seed), uint32_t(pixel * spp + s));
-          (void)simple_sample<true, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, &adj, &alt, &G);
-          continue;
-        }
-        path_sample<1, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
-        if (!any_nonzero(L[0])) continue;
-        AdjState adj{dL, L[0]};
-        rng = rng0;
-        V3 L2[1];
-        path_sample<1, true>(S, I, rng, ray, L2, stack, cnt, &adj, &G);
-      }
+      for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s)
+        adjoint_one_sample<INTEG, F>(S, I, sen, seed, px, py, pixel, spp, s, dL, G, stack, cnt);
     }
   }
 }
@@ -200,16 +209,15 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
   }
 }
 
-// volpathsimple's adjoint as a persistent launch (see
-// adjoint_persistent_kernel): lanes pull work units (a pixel's chunk of
-// camera samples) from a device counter and run each sample's primal pass and
-// its adjoint with the same stream plus the gradient-only stream, so long
-// pixels do not hold their wavefronts at the end of the launch. The result
-// per sample is that of adjoint_kernel<VOLPATH_SIMPLE>; only the order of the
-// gradient scatter-adds differs.
-template <int F>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((
-    amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH_SIMPLE>))) void adjoint_simple_persistent_kernel(
+// The per-sample integrators' adjoint (volpathsimple, surface path tracer) as
+// a persistent launch (see adjoint_persistent_kernel): lanes pull work units
+// (a pixel's chunk of camera samples) from a device counter and run
+// adjoint_one_sample for each, so long pixels do not hold their wavefronts at
+// the end of the launch. The result per sample is that of adjoint_kernel;
+// only the order of the gradient scatter-adds differs.
+template <int INTEG, int F>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void
+adjoint_sample_persistent_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2, const float *__restrict__ grad_img,
     GradTargets G, unsigned *queue, unsigned total_units) {
   extern __shared__ int lds_stack[];  // kStackSize ints per thread
@@ -217,7 +225,6 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
   LocalCounters cnt{S.counters};
-  const uint32_t alt_seed = simple_alt_seed(seed);
   int s = 0, s_end = 0, px = 0, py = 0;
   int64_t pixel = 0;
   V3 dL = v3(0.f);
@@ -244,15 +251,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((
         continue;
       }
     }
-    Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
-    const Ray ray = camera_ray(sen, px, py, rng);
-    const Pcg32 rng0 = rng;
-    const V3 L =
-        simple_sample<false, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, nullptr, nullptr, nullptr);
-    AdjState adj{dL, L};
-    rng = rng0;
-    Pcg32 alt = lane_rng(alt_seed, uint32_t(pixel * spp + s));
-    (void)simple_sample<true, F>(S, I, 0, rng, ray, simple_camera_start(), stack, cnt, &adj, &alt, &G);
+    adjoint_one_sample<INTEG, F>(S, I, sen, seed, px, py, pixel, spp, s, dL, G, stack, cnt);
     ++s;
   }
 }
@@ -572,18 +571,20 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   const int block = d->block_size ? d->block_size : kDefaultBlock;
   const unsigned grid = ceil_div(n_pix << cl2, block);
   if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
+  // the per-sample integrators' persistent adjoint, as the volume path
+  // tracer's below
+  auto launch_sample_persistent = [&](auto kern) {
+    const int ucl2 = unit_chunk_log2(spp, cl2);
+    const unsigned units = unsigned(n_pix << ucl2);
+    const unsigned pgrid =
+        std::min(resident_grid(reinterpret_cast<const void *>(kern), block), ceil_div(int64_t(units), block));
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
+    hipLaunchKernelGGL(kern, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp, ucl2,
+                       grad_img, G, s->work_queue, units);
+  };
   if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE && !d->samples_per_thread)
     with_volume_features(s->features(), [&](auto f) {
-      // as the volume path tracer's persistent adjoint below
-      constexpr int F = decltype(f)::value;
-      const int ucl2 = unit_chunk_log2(spp, cl2);
-      const unsigned units = unsigned(n_pix << ucl2);
-      const unsigned pgrid =
-          std::min(resident_grid(reinterpret_cast<const void *>(adjoint_simple_persistent_kernel<F>), block),
-                   ceil_div(int64_t(units), block));
-      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
-      hipLaunchKernelGGL(adjoint_simple_persistent_kernel<F>, dim3(pgrid), dim3(block), stack_bytes(block), stream,
-                         s->view, I, sensor, seed, spp, ucl2, grad_img, G, s->work_queue, units);
+      launch_sample_persistent(adjoint_sample_persistent_kernel<RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>);
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE)
     with_volume_features(s->features(), [&](auto f) {
@@ -610,6 +611,8 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), dim3(grid), dim3(block),
                          stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);
     });
+  else if (!d->samples_per_thread)
+    launch_sample_persistent(adjoint_sample_persistent_kernel<RCV_INTEGRATOR_PATH, 0>);
   else
     hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_PATH, 0>), dim3(grid), dim3(block), stack_bytes(block), stream,
                        s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);
