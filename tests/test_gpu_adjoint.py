@@ -227,3 +227,37 @@ def test_adjoint_equals_finite_difference_with_common_random_numbers(cuda, which
         g = plain.adjoint(sc, p, gimg, 0, 40 + s, 64)[key]
         ad = float((g.double() * torch.from_numpy(delta).cuda().double()).sum())
         assert abs(fd - ad) <= 0.01 * abs(fd), (s, fd, ad)
+
+
+@pytest.mark.parametrize("kind", ["prb", "volpathsimple"])
+def test_persistent_adjoint_equals_static_launch(cuda, kind):
+    # The per-sample adjoints (surface path tracer, volpathsimple) run as a
+    # persistent launch by default and as a static one-thread-per-chunk launch
+    # with samples_per_thread > 0; both trace the same samples, so the
+    # gradients differ only by the order of the float scatter-adds.
+    announce(f"persistent vs static adjoint launch ({kind})")
+    from recursive_control_variates_amd import scenes
+    rng = np.random.default_rng(9)
+    if kind == "prb":
+        sc = rcv.load_dict(scenes.cornell_scene_dict(scenes.texture_target(16, 1), twostate=False, film_res=40,
+                                                     spp=32))
+        key = scenes.TEXTURE_KEY
+        integ = rcv.load_dict({"type": "prb", "max_depth": 5, "rr_depth": 3})
+    else:
+        rng2, grid, sc = _setup(9, w=40, h=30)
+        key = KEY
+        integ = rcv.load_dict({"type": "volpathsimple", "max_depth": 8, "rr_depth": 4})
+    h, w = sc.film_shape(0)[:2]
+    gimg = torch.from_numpy(rng.uniform(0.5, 1.5, (h, w, 3)).astype(np.float32)).cuda()
+    p = rcv.traverse(sc)
+    p[key] = p[key].detach().clone().requires_grad_(True)
+    p.update()
+    p.update()
+    grads = []
+    for spt in (0, 4):
+        integ.samples_per_thread = spt
+        grads.append(integ.adjoint(sc, p, gimg, 0, 17, 32)[key].double().cpu().numpy())
+    a, b = grads
+    scale = np.abs(b).max()
+    assert scale > 0
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), np.abs(a - b).max() / scale
