@@ -87,12 +87,14 @@ class Adam:
         self.variables[key] = value.requires_grad_(True)
 
     def set_learning_rate(self, lr):
+        """A dict sets per-key rates; a number sets the default rate only,
+        and keys given their own rate keep it (mi.ad.Optimizer semantics:
+        a scalar replaces ``lr_default``, per-key entries stay)."""
         if isinstance(lr, dict):
             for k, v in lr.items():
                 self.lr_k[k] = float(v)
         else:
             self.lr = float(lr)
-            self.lr_k.clear()
 
     def set_clamp(self, key, lo, hi):
         self.clamp_k[key] = (float(lo), float(hi))

@@ -17,6 +17,11 @@ checkpoint images with ``save`` (``img_inf``, ``img``, ``weights``,
 the final parameters (``.vol`` grids, ``.exr`` textures, ``final_state.json``
 for other values, utils.py:45-60).
 
+``result["time_to_threshold_s"]`` is the wall clock from the start of the
+loop to the end of the step (render, loss, checkpoint, backward, Adam) whose
+updated parameters first reach ``mse_threshold``; it includes checkpoint
+renders and the per-step MSE synchronisation.
+
 ``result["runtime"]`` holds, for steps i > 16 with ``benchmark``, the GPU-
 synchronised [primal + loss, backward + step] milliseconds
 (optimize.py:47,105-132); ``step_ms`` the same two phases summed for every
@@ -184,6 +189,23 @@ def initialize_parameters(cfg, opt, scene_params):
         scene_params.update(opt)
 
 
+def _check_supported(cfg):
+    """Keys of the reference's config that change the optimisation and that
+    this driver does not implement raise instead of being ignored: the
+    large-steps preconditioner (per-variable ``largesteps``, utils.py:167-170,
+    largesteps.py) and the display-resolution render ``final_res``
+    (optimize.py:84-87, utils.py:113-127). ``denoise`` reaches
+    MetaIntegrator, which raises for it."""
+    if "final_res" in cfg:
+        raise ValueError("run_opt: 'final_res' (display-resolution renders) is not supported by this build")
+    for key, var in cfg.get("variables", {}).items():
+        if isinstance(var, dict) and var.get("largesteps"):
+            raise ValueError(f"run_opt: variable {key!r} asks for 'largesteps' preconditioning, "
+                             "which this build does not implement")
+    if cfg.get("loss", "L1") == "VGG":
+        raise ValueError("run_opt: the VGG loss is not part of this build (L1, L2)")
+
+
 def run_opt(cfg: Dict) -> Dict:
     scene = cfg["scene"]
     loss_func = _loss_func(cfg.get("loss", "L1"))
@@ -198,9 +220,10 @@ def run_opt(cfg: Dict) -> Dict:
     mse_threshold = cfg.get("mse_threshold")
     if "integrator" not in cfg:
         raise ValueError("No integrator specified")
+    _check_supported(cfg)
     base = cfg["integrator"]
     integrator = MetaIntegrator({
-        "method": method, "integrator": base,
+        "method": method, "denoise": cfg.get("denoise", False), "integrator": base,
         "adjoint_integrator": cfg.get("adjoint_integrator", base),
         "beta1": cfg.get("beta1", 0.9), "beta2": cfg.get("beta2", 0.999),
         "force_baseline": cfg.get("force_baseline", False), "pre_update": cfg.get("pre_update", False),
@@ -278,7 +301,7 @@ def run_opt(cfg: Dict) -> Dict:
                     cnt += d.numel()
             result["param_mse"][i] = mse / cnt
             if mse_threshold is not None and result["time_to_threshold_s"] is None and mse / cnt <= mse_threshold:
-                result["time_to_threshold_s"] = t1 - t_start
+                result["time_to_threshold_s"] = t3 - t_start
                 result["steps_to_threshold"] = i + 1
     params.update(opt)
     if out_dir:

@@ -152,7 +152,7 @@ def test_adam_host_state():
     opt.set_learning_rate({"a": 0.5})
     assert opt.lr_k["a"] == 0.5
     opt.set_learning_rate(0.2)
-    assert opt.lr == 0.2 and not opt.lr_k
+    assert opt.lr == 0.2 and opt.lr_k == {"a": 0.5}
     opt["a"] = torch.ones(3)        # same shape: state kept
     opt["a"] = torch.ones(4)        # new shape: state reset
     assert opt.state["a"][0].shape == (4,)
@@ -206,3 +206,51 @@ def test_cube_shapes_carry_an_analytic_box():
     obj = np.array([[1, -1, -1], [1, 1, -1], [1, 1, 1], [1, -1, 1]], np.float64)
     np.testing.assert_allclose(obj @ box[:, :3].T.astype(np.float64) + box[:, 3], v[:4], rtol=1e-6, atol=1e-6)
     assert by_name["b"].box is None and by_name["c"].box is None
+
+
+def test_adam_scalar_rate_keeps_per_key_rates():
+    # mi.ad.Optimizer: a scalar set_learning_rate replaces the default rate
+    # only; a key given its own rate (lr_factor, utils.py:82-83) keeps it
+    # through the schedule's halvings (optimize.py:139-141)
+    import torch
+    from recursive_control_variates_amd.optim import Adam
+    lr = 0.01
+    opt = Adam(lr=lr)
+    opt["a"] = torch.zeros(2)
+    opt["b"] = torch.zeros(2)
+    opt.set_learning_rate({"a": 2 * lr})
+    opt.set_learning_rate(lr / 2)
+    assert opt.lr_k.get("a", opt.lr) == 2 * lr
+    assert opt.lr_k.get("b", opt.lr) == lr / 2
+
+
+def test_integration_binding_matches_native_structs():
+    # the reference-side ctypes binding shown in INTEGRATION.md declares the
+    # same IntegratorDesc fields as the package's binding (and rcv.h)
+    import ctypes as C
+    import re
+    from recursive_control_variates_amd import _native
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"class IntegratorDesc\(C\.Structure\):\n(.*?)\n\n", text, re.S)
+    assert m, "INTEGRATION.md lost its IntegratorDesc binding"
+    ns = {"C": C}
+    exec("class IntegratorDesc(C.Structure):\n" + m.group(1), ns)
+    shown = [(n, t) for n, t in ns["IntegratorDesc"]._fields_]
+    assert shown == list(_native.IntegratorDesc._fields_)
+    hdr = open(os.path.join(ROOT, "include", "rcv.h")).read()
+    body = re.search(r"typedef struct rcv_integrator_desc \{(.*?)\} rcv_integrator_desc;", hdr, re.S).group(1)
+    names = re.findall(r"int32_t (\w+);", body)
+    assert names == [n for n, _ in shown]
+
+
+def test_run_opt_rejects_unsupported_keys():
+    from recursive_control_variates_amd.optimize import run_opt
+    base = {"scene": object(), "integrator": object(), "variables": {"k": {"init": 0.5}}}
+    with pytest.raises(ValueError, match="final_res"):
+        run_opt(dict(base, final_res=256))
+    with pytest.raises(ValueError, match="largesteps"):
+        run_opt(dict(base, variables={"k": {"init": 0.5, "largesteps": object()}}))
+    with pytest.raises(ValueError, match="VGG"):
+        run_opt(dict(base, loss="VGG"))
+    with pytest.raises(NotImplementedError):
+        run_opt(dict(base, denoise=True))

@@ -154,3 +154,57 @@ def test_cv_state_warmup_and_order():
     # pre-update: weights at it = 2 use n = 2 statistics (non-zero variance)
     assert pre.stats.n == 2
     assert np.any(pre.w_s != 0)
+
+
+def run_var(xs, b1=0.9, b2=0.999):
+    est = W.WelfordVarianceEstimatorRef(np.shape(xs[0]), b1, b2)
+    for x in xs:
+        est.update(np.asarray(x, f32))
+    return est
+
+
+@pytest.mark.parametrize("b1,b2", [(0.9, 0.999), (0.5, 0.5), (0.3, 0.8)])
+def test_single_variable_two_sample_closed_form(b1, b2):
+    # WelfordVarianceEstimator (plugins/welford.py:4-36) after x1, x2:
+    # dx1 = x2 - x1, dx2 = b1 (x2 - x1) / (1 + b1), so the corrected variance
+    # is (x2 - x1)^2 b1 / ((1 + b1)(1 + b2)); the first update adds nothing
+    x = [np.array([1.0, -2.0, 0.25, 7.0], f32), np.array([3.0, 5.0, 0.75, 7.0], f32)]
+    est = run_var(x[:1], b1, b2)
+    # zero up to the rounding of mean / (1 - b1) against x
+    assert np.all(np.abs(est.var) <= 1e-6 * x[0] * x[0]) and est.n == 1
+    est = run_var(x, b1, b2)
+    d = (x[1] - x[0]).astype(np.float64)
+    # (the constant column is zero up to rounding of the bias-corrected mean)
+    np.testing.assert_allclose(est.get(), d * d * b1 / ((1 + b1) * (1 + b2)), rtol=1e-5, atol=5e-6)
+    # bias-corrected mean: (b1 x1 + x2) / (1 + b1)
+    np.testing.assert_allclose(est.mean / W.bias_correction(b1, 2), (b1 * x[0] + x[1]) / (1 + b1), rtol=1e-6)
+    assert est.n == 2
+
+
+@pytest.mark.parametrize("b1,b2", [(0.9, 0.999), (0.5, 0.5)])
+def test_single_variable_estimator_equals_the_pair_estimator(b1, b2):
+    # the single-variable estimator is the x-half of StatisticsEstimator
+    # (welford.py:27-33 vs :59-73): on one stream both give the same mean and
+    # variance bit for bit, and get() the same corrected variance
+    rng = np.random.default_rng(11)
+    xs = rng.normal(0.3, 2.0, size=(23, 40)).astype(f32)
+    ys = rng.normal(-1.0, 0.5, size=xs.shape).astype(f32)
+    one = run_var(xs, b1, b2)
+    pair = run_stats(xs, ys, b1, b2)
+    assert one.n == pair.n == len(xs)
+    assert np.array_equal(one.mean.view(np.uint32), pair.mean_x.view(np.uint32))
+    assert np.array_equal(one.var.view(np.uint32), pair.var_x.view(np.uint32))
+    assert np.array_equal(one.get().view(np.uint32), pair.get()[0].view(np.uint32))
+
+
+def test_single_variable_constant_and_shift():
+    # constant input: zero variance at every step; a shifted stream has the
+    # same variance (to rounding) as the original
+    c = np.array([0.25, 0.5, 2.0], f32)
+    est = W.WelfordVarianceEstimatorRef((3,), 0.5, 0.5)
+    for _ in range(7):
+        est.update(c)
+        assert np.all(est.var == 0)
+    rng = np.random.default_rng(12)
+    xs = rng.normal(0, 1, size=(30, 32)).astype(f32)
+    np.testing.assert_allclose(run_var(xs).get(), run_var(xs + f32(5.0)).get(), rtol=3e-3, atol=1e-4)
