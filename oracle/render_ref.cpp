@@ -770,6 +770,85 @@ static void grid_scatter(const Scene<R> &S, int param, int med, const V<R> &p, d
   }
 }
 
+// NEW-state albedo of medium `med` at p.
+template <typename R>
+static V<R> albedo_at(const Scene<R> &S, int med, const V<R> &p) {
+  return S.s.med_albedo[med] >= 0 ? S.grid(S.s.med_albedo[med], 1, med, p)
+                                  : V<R>(S.s.med_albedo_rgb[3 * med], S.s.med_albedo_rgb[3 * med + 1],
+                                         S.s.med_albedo_rgb[3 * med + 2]);
+}
+
+// Adjoint of a colour parameter (a medium's albedo grid, a diffuse BSDF's
+// bitmap reflectance) of the NEW state by a recorded primal pass. A real
+// collision's throughput weight sigma_s/sigma_t is the albedo there, and a
+// diffuse vertex multiplies its NEE contribution and everything sampled after
+// it by the reflectance rho there; so the path's radiance is linear in that
+// value and, per channel, dL/drho = (radiance gathered from that vertex on) /
+// rho. The pass records each such vertex (its lookup and the radiance gathered
+// before it); once the path's radiance L is known, (L - L_before) / rho times
+// dLoss/dL is added to the lookup's footprint. (A value rho = 0 stops the
+// radiance behind it, and its derivative is left at zero, the convention of
+// dividing by the detached weight.)
+template <typename R>
+struct ColourTape {
+  struct Rec {
+    int med;    // >= 0: albedo grid lookup at p in this medium; -1: texture at (u, v)
+    V<R> p;
+    R u = 0, v = 0;
+    V<R> rho, L_before;
+  };
+  int param = -1;
+  std::vector<Rec> recs;
+};
+
+// Adds value (3 channels) times the bilinear weights of the texture lookup
+// at (u, v) (wrap addressing, as Scene::texture).
+template <typename R>
+static void texture_scatter(const Scene<R> &S, int param, R u, R v, const V<R> &value, std::vector<double> &g) {
+  const int nx = S.s.param_dims[4 * param], ny = S.s.param_dims[4 * param + 1];
+  R x = u * R(nx) - R(0.5), y = v * R(ny) - R(0.5);
+  R fx = std::floor(x), fy = std::floor(y);
+  R tx = x - fx, ty = y - fy;
+  auto wrap = [](long i, long n) {
+    long m = i % n;
+    return m < 0 ? m + n : m;
+  };
+  long x0 = wrap(long(fx), nx), x1 = wrap(long(fx) + 1, nx);
+  long y0 = wrap(long(fy), ny), y1 = wrap(long(fy) + 1, ny);
+  long id[4] = {y0 * nx + x0, y0 * nx + x1, y1 * nx + x0, y1 * nx + x1};
+  R w[4] = {(R(1) - tx) * (R(1) - ty), tx * (R(1) - ty), (R(1) - tx) * ty, tx * ty};
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 3; ++c) g[size_t(3 * id[k] + c)] += double(value[c]) * double(w[k]);
+}
+
+// Adds value (3 channels) times the trilinear weights of the lookup at p of a
+// 3-channel grid over medium `med`'s box.
+template <typename R>
+static void grid3_scatter(const Scene<R> &S, int param, int med, const V<R> &p, const V<R> &value,
+                          std::vector<double> &g) {
+  for (int c = 0; c < 3; ++c) {
+    std::vector<double> one(g.size() / 3, 0.0);
+    grid_scatter(S, param, med, p, double(value[c]), one);
+    for (size_t i = 0; i < one.size(); ++i) g[3 * i + c] += one[i];
+  }
+}
+
+// Scatters a finished path's taped colour gradients (dL = dLoss/dL of the
+// camera sample, L = its NEW-state radiance).
+template <typename R>
+static void tape_scatter(const Scene<R> &S, const ColourTape<R> &tape, const V<R> &dL, const V<R> &L,
+                         std::vector<double> &g) {
+  for (const auto &r : tape.recs) {
+    R gc[3];
+    for (int c = 0; c < 3; ++c) gc[c] = r.rho[c] != 0 ? dL[c] * (L[c] - r.L_before[c]) / r.rho[c] : R(0);
+    const V<R> gv(gc[0], gc[1], gc[2]);
+    if (r.med >= 0)
+      grid3_scatter(S, tape.param, r.med, r.p, gv, g);
+    else
+      texture_scatter(S, tape.param, r.u, r.v, gv, g);
+  }
+}
+
 // sample_emitter_twostates (plugins/twostatevolpath.py:387-479). With `gadj`
 // set, the pass also scatters d(log T_new)/d(density) * gadj->nee_scale.
 template <typename R>
@@ -840,7 +919,7 @@ static void sample_emitter_twostates(const Scene<R> &S, const V<R> &ref_p, const
 // TwoStatePRBVolpathIntegrator.sample_twostates (plugins/twostatevolpath.py:141-385)
 template <typename R>
 static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &rng, Ray<R> ray, V<R> L[2],
-                              Adj<R> *adj = nullptr) {
+                              Adj<R> *adj = nullptr, ColourTape<R> *tape = nullptr) {
   using DS = typename Scene<R>::DS;
   const R inf = std::numeric_limits<R>::infinity();
   const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
@@ -911,6 +990,14 @@ static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg3
       R den = mei.sigma_t[1] + mei.sigma_t[0];
       w_new = w_new * (mei.sigma_s[1] * R(2) / den);
       w_old = w_old * (mei.sigma_s[0] * R(2) / den);
+      if (tape && S.s.med_albedo[medium] == tape->param) {
+        typename ColourTape<R>::Rec r;
+        r.med = medium;
+        r.p = mei.p;
+        r.rho = albedo_at(S, medium, mei.p);
+        r.L_before = L_new;
+        tape->recs.push_back(r);
+      }
     }
     if (active_medium) {
       thr_new = thr_new * w_new;
@@ -1048,7 +1135,8 @@ static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg3
 
 // TwoStatePRBIntegrator.sample_twostates (plugins/twostatepath.py:225-364)
 template <typename R>
-static void path_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &rng, Ray<R> ray, V<R> L[2]) {
+static void path_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &rng, Ray<R> ray, V<R> L[2],
+                           ColourTape<R> *tape = nullptr) {
   using DS = typename Scene<R>::DS;
   const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
   V<R> L_old, L_new, b_old(1), b_new(1);
@@ -1077,6 +1165,15 @@ static void path_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg32 &
     if (active_next && smooth) {
       rho_new = S.reflectance(bsdf, 1, si.u, si.v);
       rho_old = S.reflectance(bsdf, 0, si.u, si.v);
+      if (tape && S.s.bsdf_tex[bsdf] == tape->param) {
+        typename ColourTape<R>::Rec r;
+        r.med = -1;
+        r.u = si.u;
+        r.v = si.v;
+        r.rho = rho_new;
+        r.L_before = L_new;
+        tape->recs.push_back(r);
+      }
     }
     {
       R u1 = R(rng.next_1d()), u2 = R(rng.next_1d());
@@ -1166,13 +1263,6 @@ struct SimpleAdj {
   std::vector<double> *g = nullptr;
   Pcg32 *alt = nullptr;
 };
-
-template <typename R>
-static V<R> albedo_at(const Scene<R> &S, int med, const V<R> &p) {
-  return S.s.med_albedo[med] >= 0 ? S.grid(S.s.med_albedo[med], 1, med, p)
-                                  : V<R>(S.s.med_albedo_rgb[3 * med], S.s.med_albedo_rgb[3 * med + 1],
-                                         S.s.med_albedo_rgb[3 * med + 2]);
-}
 
 template <typename R>
 static bool medium_interval(const Scene<R> &S, int med, const Ray<R> &ray, R &mint, R &maxt) {
@@ -1485,13 +1575,26 @@ static Ray<R> camera_ray(const OracleScene &s, int px, int py, Pcg32 &rng) {
   return r;
 }
 
-// Path-replay adjoint of the NEW state for one density parameter: primal pass
-// for L, replay with the same random numbers; dimage/dL = 1/spp (box filter).
+// Whether `param` is some medium's density grid.
+static bool is_density_param(const OracleScene &sc, int param) {
+  for (int m = 0; m < sc.n_media; ++m)
+    if (sc.med_sigma[m] == param) return true;
+  return false;
+}
+
+// Adjoint of the NEW state for one parameter; dimage/dL = 1/spp (box
+// filter). A density grid: path replay (primal pass for L, replay with the
+// same random numbers). A colour parameter (albedo grid of the volume
+// integrator, bitmap reflectance of the surface integrator): the recorded
+// primal pass of ColourTape. volpathsimple (type 2): its DRT adjoint of the
+// density.
 template <typename R>
 static void adjoint(const OracleScene &sc, const OracleIntegrator &I, uint32_t seed, int spp, const double *grad_img,
                     int param, double *grad_out, int n_threads) {
   Scene<R> S(sc);
-  const size_t n = size_t(sc.param_dims[4 * param]) * sc.param_dims[4 * param + 1] * sc.param_dims[4 * param + 2];
+  const size_t n = size_t(sc.param_dims[4 * param]) * sc.param_dims[4 * param + 1] * sc.param_dims[4 * param + 2] *
+                   sc.param_dims[4 * param + 3];
+  const bool colour = I.type != 2 && !is_density_param(sc, param);
   const int64_t n_pix = int64_t(sc.film_w) * sc.film_h;
   std::atomic<int64_t> next(0);
   n_threads = std::max(1, n_threads);
@@ -1525,6 +1628,16 @@ static void adjoint(const OracleScene &sc, const OracleIntegrator &I, uint32_t s
           a.alt = &alt;
           rng = rng0;
           (void)simple_sample(S, I, rng, ray, st, &a);
+          continue;
+        }
+        if (colour) {
+          ColourTape<R> tape;
+          tape.param = param;
+          if (I.type == 1)
+            volpath_twostates(S, I, rng, ray, L, (Adj<R> *)nullptr, &tape);
+          else
+            path_twostates(S, I, rng, ray, L, &tape);
+          tape_scatter(S, tape, dL, L[1], bufs[tid]);
           continue;
         }
         volpath_twostates(S, I, rng, ray, L);
@@ -1620,10 +1733,25 @@ int oracle_render(const OracleScene *s, const OracleIntegrator *I, uint32_t seed
   return 0;
 }
 
-// Volume adjoint (NEW state == param_new) into grad_out (grid-sized doubles).
+// Adjoint of the NEW state (== param_new) with respect to parameter `param`
+// into grad_out (doubles shaped like the parameter, channels included):
+// volume integrator (type 1) for a density or albedo grid, surface
+// integrator (type 0) for a bitmap reflectance, volpathsimple (type 2) for a
+// density grid.
 int oracle_adjoint(const OracleScene *s, const OracleIntegrator *I, uint32_t seed, int spp, int precision,
                    const double *grad_img, int param, double *grad_out, int n_threads) {
-  if (!s || !I || spp <= 0 || !grad_img || !grad_out || param < 0 || param >= s->n_params || I->type < 1) return -1;
+  if (!s || !I || spp <= 0 || !grad_img || !grad_out || param < 0 || param >= s->n_params) return -1;
+  if (I->type == 0) {
+    bool tex = false;
+    for (int b = 0; b < s->n_bsdfs; ++b) tex = tex || s->bsdf_tex[b] == param;
+    if (!tex) return -1;
+  } else if (I->type == 2 && !oracle::is_density_param(*s, param)) {
+    return -1;
+  } else if (I->type == 1 && !oracle::is_density_param(*s, param)) {
+    bool alb = false;
+    for (int m = 0; m < s->n_media; ++m) alb = alb || s->med_albedo[m] == param;
+    if (!alb || s->param_dims[4 * param + 3] != 3) return -1;
+  }
   if (precision == 64)
     oracle::adjoint<double>(*s, *I, seed, spp, grad_img, param, grad_out, n_threads);
   else

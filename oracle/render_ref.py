@@ -176,22 +176,24 @@ def render(desc, integrator_type, max_depth, rr_depth, seed, spp, params_old=Non
 
 def adjoint(desc, max_depth, rr_depth, seed, spp, grad_img, param=0, params=None, precision=64,
             n_threads=None, hide_emitters=False, integrator_type=1, flags=0):
-    """Volume adjoint of the NEW state, d<grad_img, img>/d param: path replay
-    of the volume integrator (integrator_type 1) or volpathsimple's
-    differential-ratio-tracking adjoint (2)."""
+    """Adjoint of the NEW state, d<grad_img, img>/d param, shaped like the
+    parameter: the volume integrator (integrator_type 1) for a density grid
+    (path replay) or an albedo grid (recorded primal pass), the surface
+    integrator (0) for a bitmap reflectance (recorded primal pass), or
+    volpathsimple's differential-ratio-tracking adjoint (2) of a density."""
     if params is None:
         params = [p.init for p in desc.params]
     s, keep = make_scene(desc, params, params, 0)
     I = OracleIntegrator(int(integrator_type), int(max_depth), int(rr_depth), int(hide_emitters), int(flags))
     gi = np.ascontiguousarray(grad_img, np.float64)
-    out = np.zeros(desc.params[param].shape[:3], np.float64)
+    out = np.zeros(desc.params[param].shape, np.float64)
     nt = n_threads or min(16, os.cpu_count() or 1)
     rc = lib().oracle_adjoint(C.byref(s), C.byref(I), C.c_uint32(seed & 0xFFFFFFFF), spp, precision,
                               gi.ctypes.data, param, out.ctypes.data, nt)
     if rc != 0:
         raise RuntimeError("oracle_adjoint failed")
     del keep
-    return out[..., None]
+    return out
 
 
 def majorant_grid(desc, params, medium=0, state=1):
