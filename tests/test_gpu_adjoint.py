@@ -261,3 +261,64 @@ def test_persistent_adjoint_equals_static_launch(cuda, kind):
     scale = np.abs(b).max()
     assert scale > 0
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), np.abs(a - b).max() / scale
+
+
+def _param_index(desc, key):
+    return [p.key for p in desc.params].index(key)
+
+
+def _close_to_oracle(g_gpu, g_cpu, frac=0.97):
+    scale = np.abs(g_cpu).max()
+    assert scale > 0
+    close = np.abs(g_gpu - g_cpu) <= 1e-3 * scale + 1e-3 * np.abs(g_cpu)
+    assert close.mean() > frac, close.mean()
+    assert abs(g_gpu.sum() - g_cpu.sum()) < 0.03 * np.abs(g_cpu).sum()
+    # per channel as well: a mixed-up channel shows here
+    for c in range(g_cpu.shape[-1]):
+        a, b = g_gpu[..., c].sum(), g_cpu[..., c].sum()
+        assert abs(a - b) < 0.03 * np.abs(g_cpu[..., c]).sum(), (c, a, b)
+
+
+def test_albedo_adjoint_matches_oracle_same_samples(cuda):
+    # dL/d(albedo grid) of the HIP path-replay adjoint against the oracle's
+    # recorded-primal-pass adjoint, same random numbers (float32 both)
+    announce("albedo adjoint vs oracle fp32")
+    from recursive_control_variates_amd.transform import T
+    rng = np.random.default_rng(31)
+    n = 5
+    dens = (0.3 + rng.uniform(0, 0.7, (n, n, n, 1))).astype(np.float32)
+    alb = (np.array([0.25, 0.55, 0.9], np.float32) + rng.uniform(-0.05, 0.05, (n, n, n, 3))).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(dens, albedo={"type": "gridvolume", "data": alb, "to_world": T.scale(1.0)},
+                                        scale=5.0, w=8, h=8, spp=32))
+    key = "box.interior_medium.albedo.data"
+    gimg = rng.uniform(0.5, 1.5, (8, 8, 3))
+    integ = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    p = rcv.traverse(sc)
+    p[key] = p[key].detach().clone().requires_grad_(True)
+    p.update()
+    p.update()
+    g_gpu = integ.adjoint(sc, p, torch.from_numpy(gimg.astype(np.float32)).cuda(), 0, 37, 32)[key].cpu().numpy()
+    g_cpu = R.adjoint(sc.desc, 8, 8, 37, 32, gimg, param=_param_index(sc.desc, key), precision=32)
+    assert g_gpu.shape == g_cpu.shape == alb.shape
+    _close_to_oracle(g_gpu, g_cpu)
+
+
+def test_texture_adjoint_matches_oracle_same_samples(cuda):
+    # dL/d(back-wall texture) of the HIP surface adjoint (Cornell box of the
+    # surface workload) against the oracle's, same random numbers
+    announce("texture adjoint vs oracle fp32")
+    from recursive_control_variates_amd import scenes
+    rng = np.random.default_rng(32)
+    sc = rcv.load_dict(scenes.cornell_scene_dict(scenes.texture_target(16, 0), twostate=False, film_res=32, spp=16))
+    key = scenes.TEXTURE_KEY
+    gimg = rng.uniform(0.5, 1.5, (32, 32, 3))
+    integ = rcv.load_dict({"type": "prb", "max_depth": 4})
+    p = rcv.traverse(sc)
+    p[key] = p[key].detach().clone().requires_grad_(True)
+    p.update()
+    p.update()
+    g_gpu = integ.adjoint(sc, p, torch.from_numpy(gimg.astype(np.float32)).cuda(), 0, 41, 16)[key].cpu().numpy()
+    g_cpu = R.adjoint(sc.desc, 4, 5, 41, 16, gimg, param=_param_index(sc.desc, key), precision=32,
+                      integrator_type=0)
+    assert g_gpu.shape == g_cpu.shape == (16, 16, 3)
+    _close_to_oracle(g_gpu, g_cpu)
