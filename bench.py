@@ -9,11 +9,23 @@ batch film, 32 spp primal and 32 spp adjoint, max_depth 16, grid initialised at
 0.1, Adam lr 0.01, L1 loss, method cv_ps (two-state render + CV combine).
 
 A step = scene update (two-state pointer rotation) + two-state render + fused
-CV combine + loss/gradient + path-replay adjoint + Adam. ``value`` is
-Msamples/s = (W*H*spp primal paths + W*H*spp_grad adjoint paths) / step time.
-The JSON line also carries p50/p99 step ms, time-to-param-MSE 1e-3 for cv_ps
-and the plain (baseline) loop, the roofline of the dominant kernel and a CPU
-baseline (the float32 CPU restatement on host cores, bounded sample).
+CV combine + loss/gradient + path-replay adjoint + Adam. The workload is the
+whole 200-iteration optimisation: ``--steps K`` times its last K steps
+(bracketed by device synchronisations), the ``--warmup W`` steps before them
+run untimed, as does any earlier part of the run (none with the defaults
+K = 183, W = 17: the first 17 steps are left out as optimize.py:105 leaves
+them out of its runtime figures). ``value`` is Msamples/s = (W*H*spp primal
+paths + W*H*spp_grad adjoint paths) per timed step; p50/p99 come from HIP
+events around each timed step.
+
+The same line carries: time-to-param-MSE for cv_ps and the plain loop (both
+run for the whole optimisation, MSE recorded after every step, clock = HIP
+events from the start of step 1), the rooflines of the step's two render
+kernels (HIP events around every launch during the run; algorithmic bytes
+from counting launches replayed with the inputs of sampled timed steps), a
+CPU baseline (the float32 CPU restatement of the same two launches on the
+host's cores, bounded sample) and, as extra sections, the surface-texture,
+large-grid, batched and served workloads.
 """
 from __future__ import annotations
 
@@ -36,11 +48,20 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mse-steps", type=int, default=200, help="iterations per method for time-to-MSE (0: skip)")
+    ap.add_argument("--steps", type=int, default=200 - SKIP_STEPS,
+                    help="timed steps: the last STEPS iterations of the optimisation")
+    ap.add_argument("--warmup", type=int, default=SKIP_STEPS,
+                    help="untimed steps right before the timed ones")
+    ap.add_argument("--iterations", type=int, default=200,
+                    help="optimisation length (at least warmup + steps)")
     ap.add_argument("--mse-threshold", type=float, default=1e-3)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--reachable-threshold", type=float, default=1.25e-2,
+                    help="an MSE level both methods reach within the run (the 64^3 workload's MSE has a floor)")
+    ap.add_argument("--count-samples", type=int, default=12,
+                    help="timed steps whose launches are replayed in counting mode for the rooflines")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--extras", default="surface,large_grid,batch,serve",
+                    help="extra workload sections of the default line (comma list; empty: none)")
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--spp", type=int, default=32)
@@ -51,27 +72,28 @@ def parse():
                          "volpathsimple's differential-ratio-tracking adjoint (the reference's choice for its "
                          "volume experiments, experiments.py:131)")
     ap.add_argument("--mode", choices=("step", "batch", "serve", "diff"), default="step",
-                    help="step: the volume optimisation step (default line); batch: 64 concurrent jobs in one "
+                    help="step: the volume optimisation (default line); batch: 64 concurrent jobs in one "
                          "graph-replayed iteration; serve: continuous batching under Poisson step requests; "
                          "diff: adjoints at 128^3 density + albedo, 4 x 512^2, with finite-difference checks")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--autotune", choices=("full", "short"), default="full")
     ap.add_argument("--profile-target", action="store_true",
-                    help="run only: 3 workload steps, one two-state render, one adjoint and a 256 MiB copy "
-                         "(the calibration kernel), for counter collection")
+                    help="run only the cv_ps optimisation (ITERATIONS steps) and a 256 MiB copy (the "
+                         "calibration kernel), for counter collection")
     return ap.parse_args()
 
 
 def profile_target(args):
     import torch
     st = Stepper(args, "cv_ps")
-    for _ in range(3):
-        st.step()
     torch.cuda.synchronize()
-    st.base.render_twostates(st.sc, 0, 5000, args.spp)
-    st.params.update(st.opt)
-    g = torch.full(st.sc.film_shape(0), 1e-6, device="cuda")
-    st.adj.adjoint(st.sc, st.params, g, 0, 6000, args.spp)
+    log("[bench] profile target: reference image done")
+    for i in range(args.iterations):
+        st.step()
+        if (i + 1) % 50 == 0:
+            torch.cuda.synchronize()
+            log(f"[bench] profile target: step {i + 1}")
     x = torch.ones(64 * 1024 * 1024, device="cuda")
     y = torch.empty_like(x)
     y.copy_(x)
@@ -142,22 +164,197 @@ class Stepper:
         return float((d * d).mean())
 
 
-def time_steps(st, steps, warmup):
+SKIP_STEPS = 17   # optimize.py:105 leaves a run's first 17 steps out of its runtime figures
+
+
+class _AdjointCapture:
+    """Keeps the inputs of the adjoint launches of chosen steps (image
+    gradient, seed, spp) so that their event counts can be measured after the
+    run in separate counting launches at the same parameter state."""
+
+    def __init__(self, integ, want):
+        self.integ = integ
+        self.want = set(want)
+        self.step = -1
+        self.saved = {}
+        self._orig = integ.adjoint
+        integ.adjoint = self._adjoint
+
+    def _adjoint(self, scene, params, grad_in, sensor=0, seed=0, spp=0):
+        if self.step in self.want:
+            self.saved[self.step] = dict(grad=grad_in.detach().clone(), seed=seed, spp=spp)
+        return self._orig(scene, params, grad_in, sensor, seed, spp)
+
+    def close(self):
+        del self.integ.adjoint
+
+
+def run_optimisation(args, method, img_ref, n_iter, window=None, sample=()):
+    """Runs ``n_iter`` iterations of the workload's optimisation with a HIP
+    event at every step boundary, HIP events around every adjoint and
+    two-state launch (on their launch stream), and the parameter MSE after
+    every step written to a device buffer that is read once at the end.
+    ``window`` = (first, count) brackets steps [first, first + count)
+    (0-based) by device synchronisations and times them on the host clock.
+    The launch inputs of the steps in ``sample`` and the parameter states
+    around them are kept for counting launches afterwards."""
     import numpy as np
     import torch
-    for _ in range(warmup):
-        st.step()
+    st = Stepper(args, method, img_ref=img_ref)
+    st.base.time_launches(True)
+    if st.adj is not st.base:
+        st.adj.time_launches(True)
+    adj_ev, ts_ev = st.adj.adjoint_events, st.base.twostate_events
+    want = set(sample)
+    cap = _AdjointCapture(st.adj, want)
+    snaps = {}
+    mse = torch.zeros(n_iter, dtype=torch.float32, device="cuda")
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(n_iter + 1)]
+    wall = None
     torch.cuda.synchronize()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-    t0 = time.perf_counter()
     evs[0].record()
-    for k in range(steps):
+    for i in range(n_iter):
+        if window and i == window[0]:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        if i in want or (i + 1) in want:
+            snaps[i] = st.opt[st.key].detach().clone()     # the NEW state of step i
+        cap.step = i
         st.step()
-        evs[k + 1].record()
+        evs[i + 1].record()
+        d = st.opt[st.key].detach() - st.target
+        mse[i] = (d * d).mean()
+        if window and i == window[0] + window[1] - 1:
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    per = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(steps)])
-    return wall, per
+    cap.close()
+    run = dict(per=np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(n_iter)]),
+               cum=np.array([evs[0].elapsed_time(evs[i + 1]) for i in range(n_iter)]),
+               mse=mse.double().cpu().numpy(), wall=wall, adj_ms=np.array(adj_ev.elapsed_ms()),
+               ts_ms=np.array(ts_ev.elapsed_ms()) if method == "cv_ps" else None, snaps=snaps,
+               captured=cap.saved, method=method)
+    st.base.time_launches(False)
+    st.adj.time_launches(False)
+    return st, run
+
+
+def count_bytes(args, st, run):
+    """Algorithmic bytes of the adjoint and two-state launches of the
+    sampled steps, from device event counters in counting launches with the
+    same inputs at the same parameter state:
+    two-state: 2 states x 8 trilinear corners x 4 B per tentative collision
+    and per ratio-tracking step, 2 images x 12 B per pixel;
+    adjoint: 1 state x 8 corners x 4 B per lookup in the primal and replay
+    passes and per recorded NEE collision, 4 B per gradient scatter-add,
+    12 B per pixel of the image gradient read."""
+    sc, p, key = st.sc, st.params, st.key
+    n_pix = sc.film_shape(0)[0] * sc.film_shape(0)[1]
+    rows = {}
+    for i in sorted(run["captured"]):
+        cap = run["captured"][i]
+        new = run["snaps"][i]
+        old = run["snaps"].get(i - 1, new)
+        row = {}
+        if run["method"] == "cv_ps":
+            p[st.key] = old
+            p.update()
+            p[st.key] = new
+            p.update()
+            sc.set_counting(True)
+            st.base.render_twostates(sc, 0, i, args.spp)
+            rc = sc.read_counters()
+            sc.set_counting(False)
+            row["twostate_bytes"] = 64 * (rc["collisions"] + rc["nee_steps"]) + 24 * n_pix
+            row["twostate_counters"] = rc
+        x = new.clone().requires_grad_(True)
+        for _ in range(2):
+            p[key] = x
+            p.update()
+        sc.set_counting(True)
+        st.adj.adjoint(sc, p, cap["grad"], 0, cap["seed"], cap["spp"])
+        ac = sc.read_counters()
+        sc.set_counting(False)
+        row["adjoint_bytes"] = 32 * (ac["collisions"] + ac["nee_steps"]) + 4 * ac["grad_atomics"] + 12 * n_pix
+        row["adjoint_counters"] = ac
+        rows[i] = row
+    return rows
+
+
+def _traffic(name):
+    """HBM bytes per launch of a kernel from the committed PMC summary, with
+    the state it was measured at."""
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                d = json.load(f)
+            return d.get("bytes_per_launch"), d.get("state")
+        except Exception:
+            return None, None
+    return None, None
+
+
+def rooflines(run, counts, window):
+    """Rooflines of the step's two render kernels over the sampled steps of
+    the timed window: algorithmic bytes of those launches over their
+    event-timed durations in the run."""
+    import numpy as np
+    idx = sorted(counts)
+    first, count = window
+    out = {}
+    kinds = [("volume_adjoint", "adj_ms", "adjoint_bytes",
+              "adjoint_persistent_kernel<VOLPATH> (path replay)")]
+    if run["ts_ms"] is not None:
+        kinds.append(("volume_twostate", "ts_ms", "twostate_bytes",
+                      "render_persistent_kernel<2, VOLPATH> (two-state delta tracking)"))
+    for name, mkey, bkey, kern in kinds:
+        ms = np.array([run[mkey][i] for i in idx])
+        by = np.array([counts[i][bkey] for i in idx], np.float64)
+        ach = by.sum() / (ms.sum() * 1e-3) / 1e9
+        traffic, tstate = _traffic(name)
+        out[name] = {"bound": "hbm", "achieved": round(float(ach), 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(float(ach) / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_state": tstate,
+                     "kernel": kern, "sampled_steps": [i + 1 for i in idx],
+                     "kernel_ms_sampled_mean": round(float(ms.mean()), 4),
+                     "kernel_ms_window_mean": round(float(run[mkey][first:first + count].mean()), 4),
+                     "alg_bytes_per_launch_mean": float(by.mean())}
+    return out
+
+
+def supergrid_compare(args, st, factor=8):
+    """The step's two kernels at the current parameter state with a majorant
+    supergrid of factor ``factor`` (ceil(grid / factor)^3 cells, DDA free
+    flights) against the single global majorant: event-timed durations and
+    tentative-collision counts from the device counters. Same samples, so the
+    difference is the tracker's work."""
+    import torch
+    sc, integ, adj = st.sc, st.base, st.adj
+    n_pix = args.res * args.res * 4
+    gimg = torch.full(sc.film_shape(0), 1.0 / (n_pix * 3), device="cuda")
+    x = st.opt[st.key].detach().clone().requires_grad_(True)
+    for _ in range(2):
+        st.params[st.key] = x
+        st.params.update()
+    out = {}
+    for f in (0, factor):
+        sc.set_majorant_factor(0, f)
+        integ.render_twostates(sc, 0, 999, args.spp)
+        torch.cuda.synchronize()
+        r_ms = _event_time(lambda r: integ.render_twostates(sc, 0, 3000 + r, args.spp), args.kernel_reps)
+        a_ms = _event_time(lambda r: adj.adjoint(sc, st.params, gimg, 0, 4000 + r, args.spp), args.kernel_reps)
+        sc.set_counting(True)
+        integ.render_twostates(sc, 0, 3000, args.spp)
+        rc = sc.read_counters()
+        adj.adjoint(sc, st.params, gimg, 0, 4000, args.spp)
+        ac = sc.read_counters()
+        sc.set_counting(False)
+        out[f"factor_{f}"] = {"twostate_ms": round(r_ms, 3), "adjoint_ms": round(a_ms, 3),
+                              "twostate_collisions": rc["collisions"], "twostate_null_events": rc["null_events"],
+                              "adjoint_collisions": ac["collisions"], "adjoint_nee_steps": ac["nee_steps"]}
+    sc.set_majorant_factor(0, 0)
+    out["state"] = "the parameters after the last step of the cv_ps run"
+    return out
 
 
 def _event_time(fn, reps):
@@ -176,170 +373,72 @@ def _event_time(fn, reps):
     return sum(ts) / len(ts)
 
 
-def _traffic(name):
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
-    if os.path.exists(tpath):
-        try:
-            with open(tpath) as f:
-                return json.load(f).get("bytes_per_launch")
-        except Exception:
-            return None
-    return None
+def mse_levels(args):
+    return sorted({args.mse_threshold, args.reachable_threshold, 2e-2, 1.5e-2}, reverse=True)
 
 
-def kernel_roofline(args, st):
-    """Rooflines of the step's two render kernels, each timed alone with HIP
-    events on the launch stream; event counts come from device counters in a
-    separate launch so the counting atomics stay out of the timing. The
-    adjoint is the dominant kernel of a step."""
-    import torch
-    sc, integ, adj = st.sc, st.base, st.adj
-    n_pix = args.res * args.res * 4
-    # two-state render: 2 states x 8 trilinear corners x 4 B per tentative
-    # collision and per ratio-tracking step; 2 images x 12 B per pixel
-    integ.render_twostates(sc, 0, 999, args.spp)
-    torch.cuda.synchronize()
-    r_ms = _event_time(lambda r: integ.render_twostates(sc, 0, 1000 + r, args.spp), args.kernel_reps)
-    sc.set_counting(True)
-    integ.render_twostates(sc, 0, 1000, args.spp)
-    rc = sc.read_counters()
-    r_bytes = 64 * (rc["collisions"] + rc["nee_steps"]) + 24 * n_pix
-    # adjoint: 1 state x 8 corners x 4 B per lookup in the primal, replay and
-    # NEE-gradient passes; 4 B per gradient scatter-add; 12 B per pixel read
-    gimg = torch.full(sc.film_shape(0), 1.0 / (n_pix * 3), device="cuda")
-    st.params.update(st.opt)
-    sc.set_counting(False)
-    adj.adjoint(sc, st.params, gimg, 0, 77, args.spp)
-    torch.cuda.synchronize()
-    a_ms = _event_time(lambda r: adj.adjoint(sc, st.params, gimg, 0, 2000 + r, args.spp), args.kernel_reps)
-    sc.set_counting(True)
-    adj.adjoint(sc, st.params, gimg, 0, 2000, args.spp)
-    ac = sc.read_counters()
-    sc.set_counting(False)
-    a_bytes = 32 * (ac["collisions"] + ac["nee_steps"]) + 4 * ac["grad_atomics"] + 12 * n_pix
+def time_to_levels(run, levels, mask_mse=None):
+    """First step whose parameter MSE is at or below each level and the
+    elapsed time from the start of the first step to the end of that step
+    (HIP events on the launch stream)."""
+    mse = run["mse"] if mask_mse is None else mask_mse
     out = {}
-    akern = ("adjoint_persistent_kernel<VOLPATH> (path replay)" if adj is integ
-             else "adjoint_sample_persistent_kernel<VOLPATH_SIMPLE> (differential ratio tracking)")
-    for name, ms, by, cnt, kern in (("volume_adjoint", a_ms, a_bytes, ac, akern),
-                                    ("volume_twostate", r_ms, r_bytes, rc,
-                                     "render_persistent_kernel<2, VOLPATH> (two-state delta tracking)")):
-        ach = by / (ms * 1e-3) / 1e9
-        out[name] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": _traffic(name), "kernel": kern,
-                     "kernel_ms": round(ms, 4), "alg_bytes_per_launch": by, "counters": cnt}
-    return out
-
-
-def supergrid_compare(args, st, factor=8):
-    """The step's two kernels at the current parameter state with a majorant
-    supergrid of factor ``factor`` (ceil(grid / factor)^3 cells, DDA free
-    flights) against the single global majorant: event-timed durations and
-    tentative-collision counts from the device counters. Same samples, so the
-    difference is the tracker's work."""
-    import torch
-    sc, integ, adj = st.sc, st.base, st.adj
-    n_pix = args.res * args.res * 4
-    gimg = torch.full(sc.film_shape(0), 1.0 / (n_pix * 3), device="cuda")
-    st.params.update(st.opt)
-    out = {}
-    for f in (0, factor):
-        sc.set_majorant_factor(0, f)
-        integ.render_twostates(sc, 0, 999, args.spp)
-        torch.cuda.synchronize()
-        r_ms = _event_time(lambda r: integ.render_twostates(sc, 0, 3000 + r, args.spp), args.kernel_reps)
-        a_ms = _event_time(lambda r: adj.adjoint(sc, st.params, gimg, 0, 4000 + r, args.spp), args.kernel_reps)
-        sc.set_counting(True)
-        integ.render_twostates(sc, 0, 3000, args.spp)
-        rc = sc.read_counters()
-        adj.adjoint(sc, st.params, gimg, 0, 4000, args.spp)
-        ac = sc.read_counters()
-        sc.set_counting(False)
-        out[f"factor_{f}"] = {"twostate_ms": round(r_ms, 3), "adjoint_ms": round(a_ms, 3),
-                              "twostate_collisions": rc["collisions"], "twostate_null_events": rc["null_events"],
-                              "adjoint_collisions": ac["collisions"], "adjoint_nee_steps": ac["nee_steps"]}
-    sc.set_majorant_factor(0, 0)
-    return out
-
-
-MSE_LEVELS = (1e-3, 2e-2, 1.5e-2, 1.25e-2)
-
-
-def time_to_mse(args, img_ref):
-    """Wall-clock until the unscaled-grid MSE reaches each level, for cv_ps and
-    the plain loop (MSE checked every 5 iterations; the check synchronises)."""
-    import torch
-    levels = sorted(set(MSE_LEVELS) | {args.mse_threshold}, reverse=True)
-    out = {}
-    for method in ("cv_ps", "baseline"):
-        st = Stepper(args, method, img_ref=img_ref)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        reached = {}
-        mse = st.param_mse()
-        best = mse
-        for i in range(args.mse_steps):
-            st.step()
-            if (i + 1) % 5 == 0 or i + 1 == args.mse_steps:
-                mse = st.param_mse()
-                best = min(best, mse)
-                for lv in levels:
-                    if lv not in reached and mse <= lv:
-                        reached[lv] = (round(time.perf_counter() - t0, 3), i + 1)
-        total = time.perf_counter() - t0
-        out[method] = {"seconds": reached.get(args.mse_threshold, (None,))[0],
-                       "by_level": {f"{lv:g}": {"seconds": reached[lv][0], "steps": reached[lv][1]}
-                                    if lv in reached else None for lv in levels},
-                       "final_mse": mse, "best_mse": best, "run_seconds": round(total, 3),
-                       "ms_per_iteration": round(total * 1e3 / max(1, args.mse_steps), 3),
-                       "iterations": args.mse_steps}
-        log(f"[bench] time-to-mse {method}: {out[method]}")
-        del st
-        torch.cuda.empty_cache()
-    ratios = {}
     for lv in levels:
-        a = out["cv_ps"]["by_level"][f"{lv:g}"]
-        b = out["baseline"]["by_level"][f"{lv:g}"]
-        ratios[f"{lv:g}"] = round(b["seconds"] / a["seconds"], 3) if (a and b and a["seconds"]) else None
-    out["speedup_cv_vs_plain"] = ratios
+        hit = [i for i in range(len(mse)) if mse[i] <= lv]
+        out[f"{lv:g}"] = ({"seconds": round(float(run["cum"][hit[0]]) / 1e3, 3), "steps": hit[0] + 1}
+                          if hit else None)
     return out
 
 
-def cpu_baseline(args):
-    """The float32 CPU restatement (oracle) timed on host cores on a bounded
-    slice of the workload's pixels: two-state primal paths per second."""
+def _host_cores():
+    """Cores this process may run on: its CPU affinity, capped by a cgroup
+    CPU quota when one is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            n = min(n, max(1, int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_baseline(args, state):
+    """The float32 CPU restatement (oracle) of the step's two launches — the
+    two-state primal render and the path-replay adjoint of the density — on
+    every core this process may use, at a parameter state and image gradient
+    taken from the GPU run, over every s-th film row (s halves until about
+    ``cpu_seconds`` are spent): (primal + adjoint paths) per second, the
+    unit of ``value``."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import render_ref as R
     from recursive_control_variates_amd import scene_desc, scenes
-    target = scenes.blob_grid(args.grid, 1)
-    init = np.full_like(target, 0.1)
-    d = scene_desc.describe(scenes.volume_scene_dict(init, film_res=args.res, spp=args.spp))
-    threads = min(16, os.cpu_count() or 1)
-    n_pix_total = d.sensors[0].film_width * d.sensors[0].film_height
-    # middle band of the film (rays through the medium), growing until the budget is used;
-    # once the band is the whole film, further whole-film passes (new seeds) are added
-    n = 256
-    spent = 0.0
+    old, new, gimg, seed, seed_grad, step = state
+    d = scene_desc.describe(scenes.volume_scene_dict(new, film_res=args.res, spp=args.spp))
+    threads = _host_cores()
+    W, H = d.sensors[0].film_width, d.sensors[0].film_height
+    stride, spent = 64, 0.0
     while True:
-        b = n_pix_total // 2 - n // 2
+        rows = list(range(stride // 2, H, stride))
         t0 = time.perf_counter()
-        R.render(d, 1, args.max_depth, args.max_depth, 7, args.spp, params_old=[target], params_new=[init],
-                 precision=32, n_threads=threads, pixel_range=(b, b + n))
-        dt = time.perf_counter() - t0
-        spent += dt
-        if dt > args.cpu_seconds / 3 or spent > args.cpu_seconds or n >= n_pix_total:
+        for r in rows:
+            R.render(d, 1, args.max_depth, args.max_depth, seed, args.spp, params_old=[old], params_new=[new],
+                     precision=32, n_threads=threads, pixel_range=(r * W, (r + 1) * W))
+        tp = time.perf_counter() - t0
+        g = np.zeros_like(gimg)
+        g[rows] = gimg[rows]
+        t0 = time.perf_counter()
+        R.adjoint(d, args.max_depth, args.max_depth, seed_grad, args.spp, g, param=0, params=[new], precision=32,
+                  n_threads=threads, integrator_type=1)
+        ta = time.perf_counter() - t0
+        spent += tp + ta
+        if stride == 1 or tp + ta > args.cpu_seconds / 3 or spent > args.cpu_seconds:
             break
-        n = min(n_pix_total, n * 4)
-    passes = 1
-    while n >= n_pix_total and dt < args.cpu_seconds / 3 and spent < args.cpu_seconds:
-        t0 = time.perf_counter()
-        R.render(d, 1, args.max_depth, args.max_depth, 7 + passes, args.spp, params_old=[target],
-                 params_new=[init], precision=32, n_threads=threads, pixel_range=(0, n))
-        step = time.perf_counter() - t0
-        dt += step
-        spent += step
-        passes += 1
-    msps = passes * n * args.spp / dt / 1e6
+        stride //= 2
+    n = len(rows) * W
+    msps = 2 * n * args.spp / (tp + ta) / 1e6
     cpu = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -349,13 +448,147 @@ def cpu_baseline(args):
                     break
     except OSError:
         pass
-    return {"value": round(msps, 4), "unit": "Msamples/s (two-state primal paths)", "cores": threads,
-            "kind": "port", "sample": f"{passes} pass(es) of {n} pixels x {args.spp} spp of the 4x256^2 film "
-                                      f"({'whole film' if n >= n_pix_total else 'central band'}), "
-                                      f"two-state primal, float32, {dt:.2f}s on {cpu}"}
+    return {"value": round(msps, 4), "unit": "Msamples/s (two-state primal + adjoint paths)", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(rows)} of {H} rows (every {stride}th) of the 4 x {args.res}^2 film x {args.spp} spp, "
+                      f"two-state primal {tp:.2f}s + density adjoint {ta:.2f}s, float32, at the parameters and "
+                      f"image gradient of step {step + 1} of the GPU cv_ps run, {threads} threads on {cpu}",
+            "primal_msamples_per_s": round(n * args.spp / tp / 1e6, 4),
+            "adjoint_msamples_per_s": round(n * args.spp / ta / 1e6, 4)}
 
 
-def bench_batch(args):
+def observable_texels(res=64):
+    """Back-wall texels of the Cornell box whose centre the camera sees (the
+    sphere hides the others): texel (row j, column i) lies at
+    ((i + 0.5) / res * 2 - 1, (j + 0.5) / res * 2 - 1, -1); a texel is
+    observable when the segment from the camera at (0, 0, 3.9) misses the
+    sphere of radius 0.3 at (0.35, -0.7, 0.2). The wall lies inside the
+    field of view."""
+    import numpy as np
+    c = (np.arange(res) + 0.5) / res * 2 - 1
+    yy, xx = np.meshgrid(c, c, indexing="ij")
+    p = np.stack([xx, yy, np.full_like(xx, -1.0)], -1)
+    o = np.array([0.0, 0.0, 3.9])
+    d = p - o
+    L = np.linalg.norm(d, axis=-1, keepdims=True)
+    d = d / L
+    oc = o - np.array([0.35, -0.7, 0.2])
+    b = (d * oc).sum(-1)
+    disc = b * b - (oc * oc).sum() + 0.3 ** 2
+    t = -b - np.sqrt(np.maximum(disc, 0))
+    hit = (disc > 0) & (t > 0) & (t < L[..., 0])
+    return ~hit
+
+
+def bench_surface(args):
+    """The surface workload: 128 x 128 textured Cornell box, 64 x 64 x 3
+    back-wall texture (target U[0.1, 0.9] from seed 0, initial 0.5),
+    max_depth 4, 16 spp primal and adjoint, Adam lr 0.02, L2 image loss, 50
+    iterations of cv_ps and of the plain loop: time to texture MSE 1e-3 over
+    all texels and over the texels the camera sees, per-step ms."""
+    import numpy as np
+    import torch
+    import recursive_control_variates_amd as rcv
+    from recursive_control_variates_amd import scenes
+    from recursive_control_variates_amd.cv_integrator import MetaIntegrator
+    from recursive_control_variates_amd.optim import Adam, l2_loss
+    from recursive_control_variates_amd.optimize import render_reference
+    from recursive_control_variates_amd.rendering import render
+    target = scenes.texture_target(64, 0)
+    init = np.full_like(target, 0.5)
+    obs = torch.from_numpy(observable_texels(64)).cuda()
+    tgt = torch.from_numpy(target).cuda()
+    levels = (1e-2, 5e-3, 2e-3, 1e-3)
+    out = {"workload": "Cornell box 128^2, 64^2 x 3 texture, max_depth 4, spp 16/16, Adam lr 0.02, L2, "
+                       "50 iterations", "observable_texels": int(obs.sum()), "texels": 64 * 64}
+    img_ref = None
+    for method in ("cv_ps", "baseline"):
+        sc = rcv.load_dict(scenes.cornell_scene_dict(init, twostate=(method == "cv_ps"), film_res=128, spp=16))
+        base = rcv.load_dict({"type": "twostateprb" if method == "cv_ps" else "prb", "max_depth": 4})
+        if img_ref is None:
+            plain = rcv.load_dict({"type": "prb", "max_depth": 4})
+            img_ref = render_reference({"ref_passes": 4, "spp_ref": 1024}, sc, plain,
+                                       target={scenes.TEXTURE_KEY: tgt})
+        meta = MetaIntegrator({"method": method, "integrator": base})
+        p = rcv.traverse(sc)
+        opt = Adam(lr=0.02)
+        opt.set_clamp(scenes.TEXTURE_KEY, 0.0, 1.0)
+        for _ in range(2 if method == "cv_ps" else 1):
+            opt[scenes.TEXTURE_KEY] = torch.from_numpy(init).cuda()
+            p.update(opt)
+        n = 50
+        mse = torch.zeros(n, device="cuda")
+        mse_obs = torch.zeros(n, device="cuda")
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+        torch.cuda.synchronize()
+        evs[0].record()
+        for i in range(n):
+            p.update(opt)
+            img = render(sc, p, integrator=meta, seed=i, spp=16, spp_grad=16)
+            l2_loss(img, img_ref).backward()
+            opt.step()
+            evs[i + 1].record()
+            d2 = (opt[scenes.TEXTURE_KEY].detach() - tgt) ** 2
+            mse[i] = d2.mean()
+            mse_obs[i] = d2[obs].mean()
+        torch.cuda.synchronize()
+        run = {"cum": np.array([evs[0].elapsed_time(evs[i + 1]) for i in range(n)]),
+               "per": np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(n)]),
+               "mse": mse.double().cpu().numpy()}
+        mo = mse_obs.double().cpu().numpy()
+        out[method] = {"seconds_to_1e-3": time_to_levels(run, [1e-3])["0.001"],
+                       "seconds_to_1e-3_observable": time_to_levels(run, [1e-3], mo)["0.001"],
+                       "by_level": time_to_levels(run, levels),
+                       "by_level_observable": time_to_levels(run, levels, mo),
+                       "final_mse": float(run["mse"][-1]), "final_mse_observable": float(mo[-1]),
+                       "ms_per_step_mean": round(float(run["per"].mean()), 4),
+                       "p50_step_ms": round(float(np.percentile(run["per"], 50)), 4),
+                       "p99_step_ms": round(float(np.percentile(run["per"], 99)), 4)}
+        log(f"[bench] surface {method}: {out[method]}")
+    return out
+
+
+def bench_large_grid(args):
+    """Timings at 128^3 density + 128^3 x 3 albedo grid, 4 views of 512^2,
+    32 spp, max_depth 16: the two-state render and both adjoints, with one
+    global majorant and with the k = 8 supergrid."""
+    import torch
+    import recursive_control_variates_amd as rcv
+    from recursive_control_variates_amd import scenes
+    res, grid, spp = 512, 128, 32
+    dens = scenes.blob_grid(grid, 1)
+    alb = scenes.albedo_grid(grid, 1)
+    sc = rcv.load_dict(scenes.volume_scene_dict(dens, film_res=res, spp=spp, albedo_grid_data=alb))
+    p = rcv.traverse(sc)
+    ts = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 16, "rr_depth": 16})
+    prb = rcv.load_dict({"type": "prbvolpath", "max_depth": 16, "rr_depth": 16})
+    simple = rcv.load_dict({"type": "volpathsimple", "max_depth": 16, "rr_depth": 16})
+    n_pix = res * res * 4
+    gimg = torch.full((res, 4 * res, 3), 1.0 / (n_pix * 3), device="cuda")
+    p[scenes.VOLUME_KEY] = torch.from_numpy(dens).cuda().requires_grad_(True)
+    p[scenes.ALBEDO_KEY] = torch.from_numpy(alb).cuda().requires_grad_(True)
+    p.update()
+    p.update()
+    out = {"workload": "128^3 density + 128^3 x 3 albedo, 4 x 512^2, 32 spp, max_depth 16, d/d(density) and "
+                       "d/d(albedo) per adjoint launch", "samples_per_launch": n_pix * spp}
+    for f in (0, 8):
+        sc.set_majorant_factor(0, f)
+        ts.render_twostates(sc, 0, 1, spp)
+        prb.adjoint(sc, p, gimg, 0, 2, spp)
+        torch.cuda.synchronize()
+        row = {"twostate_ms": round(_event_time(lambda r: ts.render_twostates(sc, 0, 10 + r, spp), 3), 3),
+               "adjoint_prb_ms": round(_event_time(lambda r: prb.adjoint(sc, p, gimg, 0, 20 + r, spp), 3), 3)}
+        if f == 0:
+            simple.adjoint(sc, p, gimg, 0, 2, spp)
+            row["adjoint_volpathsimple_ms"] = round(
+                _event_time(lambda r: simple.adjoint(sc, p, gimg, 0, 20 + r, spp), 3), 3)
+        out[f"majorant_factor_{f}"] = row
+    sc.set_majorant_factor(0, 0)
+    log(f"[bench] large grid {out}")
+    return out
+
+
+def bench_batch(args, short=False):
     """Config-4 style: ``--jobs`` volume optimisations (per-job seeds 100.., image
     side U{64..512}, grid U{32,64,128}, spp U{4..64}, max_depth U{8..32}), cv_ps,
     advanced together by one hipGraph-replayed batched iteration, after an
@@ -376,13 +609,15 @@ def bench_batch(args):
     log(f"[bench] {len(made)} jobs set up in {time.perf_counter() - t0:.1f}s")
     paths = sum((m["side"] ** 2) * m["spp"] * 2 for _, m in made)
     small = lambda: [make_job(s, "volume", "cv_ps")[0] for s in seeds[:8]]
-    best, table = autotune(small, blocks=(64, 128, 256, 512), grabs=(64, 128, 256), steps=2)
+    short = short or args.autotune == "short"
+    blocks, grabs = ((64, 256), (64, 128)) if short else ((64, 128, 256, 512), (64, 128, 256))
+    best, table = autotune(small, blocks=blocks, grabs=grabs, steps=2)
     log(f"[bench] autotune best {best}: {table}")
     opt = BatchOptimizer([sp for sp, _ in made], use_graph=True, block_size=best[0], queue_grab=best[1])
     opt.step(2)
     opt.synchronize()
     t = time.perf_counter()
-    K = max(1, args.steps // 4)
+    K = 5 if short else 10
     opt.step(K)
     opt.synchronize()
     batched_ms = (time.perf_counter() - t) * 1e3 / K
@@ -421,20 +656,21 @@ def bench_batch(args):
             "hip_graph": True, "dtype": "f32", "data": "synthetic (blob targets, seeds 100..)"}
 
 
-def bench_serve(args):
+def bench_serve(args, seconds=None):
     """Config-5 style: Poisson step requests at 200/s over a 70% surface /
     30% volume job mix (sizes as config 4, 100-500 iterations per job), HBM
     admission budget 200 GB, continuous batching."""
     from recursive_control_variates_amd.serve import run_simulation
-    offered = run_simulation(duration=args.serve_seconds, rate=200.0, surface_fraction=0.7, target_jobs=16,
+    seconds = seconds or args.serve_seconds
+    offered = run_simulation(duration=seconds, rate=200.0, surface_fraction=0.7, target_jobs=16,
                              iterations=(100, 500), budget_bytes=200e9, seed=0, log=log)
     capacity = offered["served_steps"] / offered["duration_s"]
     log(f"[bench] offered 200 steps/s: served {capacity:.1f} steps/s")
     # below saturation the latency is the service time, not the queue length
     rate = max(1.0, 0.7 * capacity)
-    below = run_simulation(duration=args.serve_seconds, rate=rate, surface_fraction=0.7, target_jobs=16,
+    below = run_simulation(duration=seconds, rate=rate, surface_fraction=0.7, target_jobs=16,
                            iterations=(100, 500), budget_bytes=200e9, seed=1, log=log)
-    return {"metric": "served optimisation steps: p50/p99 step latency and Msamples/s",
+    return {"metric": "served optimisation steps: p50/p99 step latency and Msamples/s", "seconds": seconds,
             "offered_200_per_s": offered, "capacity_steps_per_s": round(capacity, 2),
             "at_70pct_capacity": below}
 
@@ -558,32 +794,86 @@ def main():
     if args.mode == "diff":
         print(json.dumps(bench_diff(args)), flush=True)
         return
-    st = Stepper(args, "cv_ps")
+    K, W = args.steps, args.warmup
+    n_iter = max(args.iterations, K + W)
+    first = n_iter - K                       # 0-based index of the first timed step
+    skip = min(SKIP_STEPS, first)
+    pick = np.unique(np.linspace(first, n_iter - 1, max(1, args.count_samples)).round().astype(int)).tolist()
+    sc_, integ_, target_, _ = build_problem(args, "cv_ps")
+    img_ref = reference_image(args, sc_, integ_, target_)
+    del sc_, integ_, target_
     log("[bench] reference image rendered")
-    wall, per = time_steps(st, args.steps, args.warmup)
-    ms = wall * 1e3 / args.steps
+    runs, states = {}, {}
+    for method in ("cv_ps", "baseline"):
+        st, run = run_optimisation(args, method, img_ref, n_iter, window=(first, K) if method == "cv_ps" else None,
+                                   sample=pick if method == "cv_ps" else ())
+        log(f"[bench] {method}: {n_iter} iterations, {run['cum'][-1] / 1e3:.2f} s; final MSE {run['mse'][-1]:.5f}")
+        runs[method] = run
+        if method == "cv_ps":
+            counts = count_bytes(args, st, run)
+            mid = pick[len(pick) // 2]
+            states["cpu"] = (run["snaps"].get(mid - 1, run["snaps"][mid]).cpu().numpy(),
+                             run["snaps"][mid].cpu().numpy(), run["captured"][mid]["grad"].cpu().numpy(),
+                             mid, run["captured"][mid]["seed"], mid)
+            sg = supergrid_compare(args, st)
+            log(f"[bench] supergrid {sg}")
+        del st
+        torch.cuda.empty_cache()
+    cv = runs["cv_ps"]
     n_pix = args.res * args.res * 4
     samples = n_pix * args.spp * 2
+    ms = cv["wall"] * 1e3 / K
     msps = samples / (ms * 1e-3) / 1e6
-    log(f"[bench] {args.steps} steps: {ms:.3f} ms/step, {msps:.1f} Msamples/s")
-    roof = kernel_roofline(args, st)
+    per = cv["per"][first:]
+    log(f"[bench] timed steps {first + 1}..{n_iter}: {ms:.3f} ms/step, {msps:.1f} Msamples/s")
+    roof = rooflines(cv, counts, (first, K))
     log(f"[bench] roofline {roof}")
-    sg = supergrid_compare(args, st)
-    log(f"[bench] supergrid {sg}")
-    img_ref = st.img_ref
-    del st
-    ttm = time_to_mse(args, img_ref) if args.mse_steps > 0 else None
-    cpu = cpu_baseline(args) if args.cpu_seconds > 0 else None
+    levels = mse_levels(args)
+    ttm = {"threshold": args.mse_threshold, "reachable_threshold": args.reachable_threshold,
+           "clock": "HIP events from the start of step 1 to the end of the step that reaches the level"}
+    for method, run in runs.items():
+        lv = time_to_levels(run, levels)
+        hit = lv[f"{args.mse_threshold:g}"]
+        rh = lv[f"{args.reachable_threshold:g}"]
+        ttm[method] = {"seconds": hit["seconds"] if hit else None,
+                       "seconds_to_reachable": rh["seconds"] if rh else None,
+                       "by_level": lv, "final_mse": float(run["mse"][-1]), "best_mse": float(run["mse"].min()),
+                       "iterations": n_iter, "run_seconds": round(float(run["cum"][-1]) / 1e3, 3),
+                       "ms_per_step_after_skip": round(float(run["per"][skip:].mean()), 4),
+                       "p50_step_ms_after_skip": round(float(np.percentile(run["per"][skip:], 50)), 4),
+                       "p99_step_ms_after_skip": round(float(np.percentile(run["per"][skip:], 99)), 4)}
+    ratios = {}
+    for lv in levels:
+        a = ttm["cv_ps"]["by_level"][f"{lv:g}"]
+        b = ttm["baseline"]["by_level"][f"{lv:g}"]
+        ratios[f"{lv:g}"] = round(b["seconds"] / a["seconds"], 3) if (a and b and a["seconds"]) else None
+    ttm["speedup_cv_vs_plain"] = ratios
+    cpu = cpu_baseline(args, states["cpu"]) if args.cpu_seconds > 0 else None
     log(f"[bench] cpu baseline {cpu}")
+    extras = {}
+    for name in [e for e in args.extras.split(",") if e]:
+        t0 = time.perf_counter()
+        if name == "surface":
+            extras["surface_texture"] = bench_surface(args)
+        elif name == "large_grid":
+            extras["large_grid"] = bench_large_grid(args)
+        elif name == "batch":
+            extras["batched_64_jobs"] = bench_batch(args, short=True)
+        elif name == "serve":
+            extras["served"] = bench_serve(args, seconds=10.0)
+        else:
+            raise SystemExit(f"unknown extra section {name!r}")
+        torch.cuda.empty_cache()
+        log(f"[bench] extra {name}: {time.perf_counter() - t0:.1f} s")
     out = {
         "metric": "Msamples/s (two-state primal + adjoint paths) of a cv_ps volume optimisation step; "
                   "also time-to-param-MSE 1e-3 (CV vs plain) and p50/p99 step ms",
-        "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (procedural blob-grid target, seed 1)",
         "config": {"workload": f"heterogeneous volume {args.grid}^3, 4 views {args.res}x{args.res}, "
                                f"spp {args.spp}/{args.spp}, max_depth {args.max_depth}, cv_ps, Adam lr 0.01, L1, "
-                               f"adjoint {args.adjoint}",
+                               f"adjoint {args.adjoint}, {n_iter} iterations; timed: steps {first + 1}..{n_iter}",
                    "global_batch": n_pix * args.spp, "parallelism": "single GPU"},
         "p50_step_ms": round(float(np.percentile(per, 50)), 4),
         "p99_step_ms": round(float(np.percentile(per, 99)), 4),
@@ -593,6 +883,7 @@ def main():
         "roofline_detail": roof,
         "majorant_supergrid": sg,
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(out), flush=True)
 

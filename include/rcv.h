@@ -40,6 +40,8 @@ extern "C" {
 /* parameter kinds */
 #define RCV_PARAM_TEXTURE 0 /* (ny, nx, channels) bitmap, bilinear, repeat */
 #define RCV_PARAM_GRID 1    /* (nz, ny, nx, channels) grid, trilinear, clamp */
+#define RCV_PARAM_VALUE 2   /* a constant of `channels` (1 or 3) values, nx = ny = nz = 1
+                               (a homogeneous medium's sigma_t / albedo) */
 
 /* bsdf types */
 #define RCV_BSDF_DIFFUSE 0
@@ -83,19 +85,26 @@ typedef struct rcv_integrator_desc {
   int32_t flags;         /* integrator-specific switches (RCV_SIMPLE_*) */
 } rcv_integrator_desc;
 
-/* Heterogeneous medium with an axis-aligned grid box (mitsuba 'heterogeneous'
- * with a 'gridvolume' sigma_t, as used by experiments.py:52-69). */
+/* A medium: heterogeneous, with an axis-aligned grid box (mitsuba
+ * 'heterogeneous' with a 'gridvolume' sigma_t, as used by
+ * experiments.py:52-69), or homogeneous (mitsuba 'homogeneous', the medium of
+ * tutorial.ipynb:77-88): constant sigma_t and albedo, unbounded. A scene holds
+ * at most one homogeneous medium. */
 typedef struct rcv_medium_desc {
-  int32_t sigma_t_param;   /* RCV_PARAM_GRID id (1 channel) */
-  int32_t albedo_param;    /* RCV_PARAM_GRID id (3 channels) or -1 */
+  int32_t sigma_t_param;   /* RCV_PARAM_GRID id (1 channel); homogeneous:
+                              RCV_PARAM_VALUE id (1 channel) */
+  int32_t albedo_param;    /* RCV_PARAM_GRID id (3 channels), homogeneous:
+                              RCV_PARAM_VALUE id (3 channels); or -1 */
   float albedo_rgb[3];     /* used when albedo_param < 0 */
-  float scale;             /* sigma_t = scale * grid */
-  float bbox_min[3];       /* world-space box the grid spans */
+  float scale;             /* sigma_t = scale * grid (or value) */
+  float bbox_min[3];       /* world-space box the grid spans (heterogeneous) */
   float bbox_max[3];
   int32_t sample_emitters; /* NEE inside the medium (default 1) */
   int32_t majorant_factor; /* 0 = one global majorant; k>0 = supergrid with
                               cells of k^3 voxels traversed by DDA
-                              (plugins/twostatemedium.py:25-70) */
+                              (plugins/twostatemedium.py:25-70); 0 when
+                              homogeneous */
+  int32_t homogeneous;     /* 1: homogeneous medium (bbox ignored) */
 } rcv_medium_desc;
 
 /* ---------------------------------------------------------------- library */

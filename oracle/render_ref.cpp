@@ -66,6 +66,7 @@ typedef struct OracleScene {
   int film_w, film_h, n_views;
   const float *views;  // origin3 left3 up3 fwd3 tan_half  (13 per view)
   const int *med_majorant_factor;  // per medium: 0 = one global majorant, k > 0 = supergrid of ceil(n / k) cells
+  const int *med_homogeneous;      // per medium: 1 = homogeneous (sigma_t / albedo params are 1 / 3 values)
 } OracleScene;
 
 typedef struct OracleIntegrator {
@@ -251,8 +252,20 @@ struct Scene {
   // both states
   std::vector<std::array<int, 3>> sg_res;
   std::vector<std::vector<R>> sg_old, sg_new;
+  // prepare_scene flags of upstream prbvolpath (plugins/twostatevolpath.py:99):
+  // null scattering when any medium is heterogeneous; analytic NEE
+  // transmittance in homogeneous media when there is one
+  bool handle_null = false, nee_homog = false;
+
+  bool homog(int m) const { return s.med_homogeneous && s.med_homogeneous[m] != 0; }
 
   explicit Scene(const OracleScene &sc) : s(sc) {
+    for (int m = 0; m < s.n_media; ++m) {
+      if (homog(m))
+        nee_homog = true;
+      else
+        handle_null = true;
+    }
     for (int t = 0; t < s.n_tris; ++t) {
       const float *v = s.tri_v + 9 * t;
       V<R> a(v[0], v[1], v[2]), b(v[3], v[4], v[5]), c(v[6], v[7], v[8]);
@@ -661,9 +674,42 @@ struct MEI {
   V<R> sigma_s[2];
 };
 
+// Albedo of medium `med` for state k (0 old, 1 new) at p.
+template <typename R>
+static V<R> albedo_state(const Scene<R> &S, int med, int k, const V<R> &p) {
+  const int ap = S.s.med_albedo[med];
+  if (ap < 0)
+    return V<R>(S.s.med_albedo_rgb[3 * med], S.s.med_albedo_rgb[3 * med + 1], S.s.med_albedo_rgb[3 * med + 2]);
+  if (S.homog(med)) {
+    const float *a = k == 0 ? S.s.param_old[ap] : S.s.param_new[ap];
+    return V<R>(R(a[0]), R(a[1]), R(a[2]));
+  }
+  return S.grid(ap, k, med, p);
+}
+
 template <typename R>
 static MEI<R> sample_interaction_twostates(const Scene<R> &S, int med, const Ray<R> &ray, R sample) {
   MEI<R> mei;
+  if (S.homog(med)) {
+    // homogeneous: prepare_interaction_sampling gives [0, ray.maxt]; the
+    // majorant of each state is its sigma_t (:71-75); constant coefficients
+    const int sp = S.s.med_sigma[med];
+    const R scale = S.s.med_scale[med];
+    for (int k = 0; k < 2; ++k) {
+      mei.sigma_t[k] = scale * R(k == 0 ? S.s.param_old[sp][0] : S.s.param_new[sp][0]);
+      mei.sigma_s[k] = albedo_state(S, med, k, V<R>()) * mei.sigma_t[k];
+    }
+    const R combined = std::max(mei.sigma_t[0], mei.sigma_t[1]);
+    const bool active = combined > 0 && std::isfinite(combined);
+    const R sampled_t = -std::log(R(1) - sample) / combined;
+    mei.valid = active && sampled_t <= ray.maxt;
+    mei.t = mei.valid ? sampled_t : std::numeric_limits<R>::infinity();
+    mei.p = ray.o + ray.d * sampled_t;
+    mei.mint = 0;
+    mei.combined = combined;
+    for (int k = 0; k < 2; ++k) mei.sigma_n[k] = combined - mei.sigma_t[k];
+    return mei;
+  }
   const float *lo = S.s.med_lo + 3 * med, *hi = S.s.med_hi + 3 * med;
   // prepare_interaction_sampling: ray / medium bounding box
   R t0 = -std::numeric_limits<R>::infinity(), t1 = std::numeric_limits<R>::infinity();
@@ -700,11 +746,7 @@ static MEI<R> sample_interaction_twostates(const Scene<R> &S, int med, const Ray
   for (int k = 0; k < 2; ++k) {
     if (valid_mi) {
       mei.sigma_t[k] = scale * S.grid(sp, k, med, mei.p).x;
-      V<R> alb = S.s.med_albedo[med] >= 0
-                     ? S.grid(S.s.med_albedo[med], k, med, mei.p)
-                     : V<R>(S.s.med_albedo_rgb[3 * med], S.s.med_albedo_rgb[3 * med + 1],
-                            S.s.med_albedo_rgb[3 * med + 2]);
-      mei.sigma_s[k] = alb * mei.sigma_t[k];
+      mei.sigma_s[k] = albedo_state(S, med, k, mei.p) * mei.sigma_t[k];
     }
     mei.sigma_n[k] = combined - mei.sigma_t[k];  // :96-97
   }
@@ -773,9 +815,7 @@ static void grid_scatter(const Scene<R> &S, int param, int med, const V<R> &p, d
 // NEW-state albedo of medium `med` at p.
 template <typename R>
 static V<R> albedo_at(const Scene<R> &S, int med, const V<R> &p) {
-  return S.s.med_albedo[med] >= 0 ? S.grid(S.s.med_albedo[med], 1, med, p)
-                                  : V<R>(S.s.med_albedo_rgb[3 * med], S.s.med_albedo_rgb[3 * med + 1],
-                                         S.s.med_albedo_rgb[3 * med + 2]);
+  return albedo_state(S, med, 1, p);
 }
 
 // Adjoint of a colour parameter (a medium's albedo grid, a diffuse BSDF's
@@ -842,7 +882,9 @@ static void tape_scatter(const Scene<R> &S, const ColourTape<R> &tape, const V<R
     R gc[3];
     for (int c = 0; c < 3; ++c) gc[c] = r.rho[c] != 0 ? dL[c] * (L[c] - r.L_before[c]) / r.rho[c] : R(0);
     const V<R> gv(gc[0], gc[1], gc[2]);
-    if (r.med >= 0)
+    if (r.med >= 0 && S.homog(r.med))
+      for (int c = 0; c < 3; ++c) g[size_t(c)] += double(gv[c]);
+    else if (r.med >= 0)
       grid3_scatter(S, tape.param, r.med, r.p, gv, g);
     else
       texture_scatter(S, tape.param, r.u, r.v, gv, g);
@@ -877,16 +919,27 @@ static void sample_emitter_twostates(const Scene<R> &S, const V<R> &ref_p, const
     bool active_medium = medium >= 0;
     bool active_surface = !active_medium;
     MEI<R> mei;
+    R m_old = 1, m_new = 1;
     if (active_medium) {
       mei = sample_interaction_twostates(S, medium, ray, R(rng.next_1d()));
       if (si.t < mei.t) {
         mei.t = std::numeric_limits<R>::infinity();
         mei.valid = false;
       }
+      if (S.nee_homog && S.homog(medium)) {
+        // :422-431 with eval_tr_old/new (plugins/twostatemedium.py:109-115)
+        const R t = std::min(remaining_dist, si.t);
+        const R seg = std::min(t, si.t) - mei.mint;
+        m_old = std::exp(-(seg * mei.sigma_t[0]));
+        m_new = std::exp(-(seg * mei.sigma_t[1]));
+        mei.t = std::numeric_limits<R>::infinity();
+        mei.valid = false;
+        if (gadj && S.s.med_sigma[medium] == gadj->param)
+          (*gadj->g)[0] += -gadj->nee_scale * double(S.s.med_scale[medium]) * double(seg);
+      }
     }
     bool escaped_medium = active_medium && !mei.valid;
     active_medium = active_medium && mei.valid;
-    R m_old = 1, m_new = 1;
     if (active_medium) {
       ray.o = mei.p;
       si.t = si.t - mei.t;
@@ -949,10 +1002,12 @@ static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg3
     }
     bool active_medium = medium >= 0;
     bool active_surface = !active_medium;
+    const bool hom = active_medium && S.homog(medium);
     MEI<R> mei;
     if (active_medium) {
       R u = R(rng.next_1d());
       mei = sample_interaction_twostates(S, medium, ray, u);
+      if (hom && mei.valid) ray.maxt = mei.t;  // (:207)
       if (needs_intersection) si = S.intersect(ray);
       needs_intersection = false;
       if (si.t < mei.t) {
@@ -964,7 +1019,26 @@ static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg3
     bool escaped_medium = active_medium && !mei.valid;
     active_medium = active_medium && mei.valid;
     bool act_null = false, act_scatter = false;
-    if (active_medium) {
+    // without null scattering (no heterogeneous medium): free-flight
+    // transmittance over the NEW state's pdf (:212-213, 232-247)
+    R ff_t = 0;
+    if (!S.handle_null && (escaped_medium || active_medium)) {
+      ff_t = std::min(mei.t, si.t) - mei.mint;
+      const R sig = mei.sigma_t[1];
+      const R tr = std::exp(-(ff_t * sig));
+      const R pdf = si.t < mei.t ? tr : tr * sig;
+      const R r_new = pdf > 0 ? std::exp(-(ff_t * mei.sigma_t[1])) / pdf : R(0);
+      const R r_old = pdf > 0 ? std::exp(-(ff_t * mei.sigma_t[0])) / pdf : R(0);
+      if (escaped_medium) {
+        thr_new = thr_new * r_new;
+        thr_old = thr_old * r_old;
+      } else {
+        w_new = w_new * r_new;
+        w_old = w_old * r_old;
+      }
+      act_scatter = active_medium;
+    }
+    if (active_medium && S.handle_null) {
       R majorant = mei.combined;
       R p_new = mei.sigma_t[1] / majorant, p_old = mei.sigma_t[0] / majorant;
       R scatter_prob = majorant != 0 ? (p_new + p_old) * R(0.5) : R(0);
@@ -987,9 +1061,14 @@ static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg3
       si.t = si.t - mei.t;
     }
     if (act_scatter) {
-      R den = mei.sigma_t[1] + mei.sigma_t[0];
-      w_new = w_new * (mei.sigma_s[1] * R(2) / den);
-      w_old = w_old * (mei.sigma_s[0] * R(2) / den);
+      if (!S.handle_null) {
+        w_new = w_new * mei.sigma_s[1];  // (:261-263)
+        w_old = w_old * mei.sigma_s[0];
+      } else {
+        R den = mei.sigma_t[1] + mei.sigma_t[0];
+        w_new = w_new * (mei.sigma_s[1] * R(2) / den);
+        w_old = w_old * (mei.sigma_s[0] * R(2) / den);
+      }
       if (tape && S.s.med_albedo[medium] == tape->param) {
         typename ColourTape<R>::Rec r;
         r.med = medium;
@@ -1003,7 +1082,25 @@ static void volpath_twostates(const Scene<R> &S, const OracleIntegrator &I, Pcg3
       thr_new = thr_new * w_new;
       thr_old = thr_old * w_old;
     }
-    if (adj && (act_null || act_scatter) && S.s.med_sigma[medium] == adj->param) {
+    if (adj && hom && (escaped_medium || active_medium) && S.s.med_sigma[medium] == adj->param) {
+      // a homogeneous medium's value (sigma_t = scale value): null collision
+      // -scale/sigma_n, real +scale/sigma_t; without null scattering the
+      // weight exp(-sigma_t t)/pdf gives -scale t and a real collision's
+      // sigma_s another +scale/sigma_t
+      const double lr = double(dot(adj->dL, adj->Lrem));
+      const double scale = S.s.med_scale[medium];
+      double coef = 0;
+      if (S.handle_null) {
+        if (act_null)
+          coef = mei.sigma_n[1] != 0 ? -scale / double(mei.sigma_n[1]) : 0.0;
+        else if (act_scatter)
+          coef = scale / double(mei.sigma_t[1]);
+      } else {
+        coef = -double(ff_t) * scale + (act_scatter ? scale / double(mei.sigma_t[1]) : 0.0);
+      }
+      (*adj->g)[0] += lr * coef;
+    }
+    if (adj && !hom && (act_null || act_scatter) && S.s.med_sigma[medium] == adj->param) {
       // d log(weight_new)/d density times everything gathered after it:
       // null: weight sigma_n/sigma_n(detached) -> -scale/sigma_n;
       // real: weight sigma_s/sigma_t(detached) -> +scale/sigma_t.

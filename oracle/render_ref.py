@@ -33,7 +33,7 @@ class OracleScene(C.Structure):
         ("med_hi", C.c_void_p), ("med_sample_emitters", C.c_void_p),
         ("n_params", C.c_int), ("param_dims", C.c_void_p), ("param_old", C.c_void_p), ("param_new", C.c_void_p),
         ("film_w", C.c_int), ("film_h", C.c_int), ("n_views", C.c_int), ("views", C.c_void_p),
-        ("med_majorant_factor", C.c_void_p),
+        ("med_majorant_factor", C.c_void_p), ("med_homogeneous", C.c_void_p),
     ]
 
 
@@ -125,10 +125,13 @@ def make_scene(desc, params_old, params_new, sensor=0):
     s.med_hi = keep.arr([m.bbox_max for m in desc.media], np.float32)
     s.med_sample_emitters = keep.arr([int(m.sample_emitters) for m in desc.media], np.int32)
     s.med_majorant_factor = keep.arr([int(m.majorant_factor) for m in desc.media], np.int32)
+    s.med_homogeneous = keep.arr([int(getattr(m, "homogeneous", False)) for m in desc.media], np.int32)
     s.n_params = len(desc.params)
     dims = []
     for p in desc.params:
-        if len(p.shape) == 3:
+        if len(p.shape) == 1:
+            dims.append([1, 1, 1, p.shape[0]])
+        elif len(p.shape) == 3:
             dims.append([p.shape[1], p.shape[0], 1, p.shape[2]])
         else:
             dims.append([p.shape[2], p.shape[1], p.shape[0], p.shape[3]])
@@ -178,7 +181,8 @@ def adjoint(desc, max_depth, rr_depth, seed, spp, grad_img, param=0, params=None
             n_threads=None, hide_emitters=False, integrator_type=1, flags=0):
     """Adjoint of the NEW state, d<grad_img, img>/d param, shaped like the
     parameter: the volume integrator (integrator_type 1) for a density grid
-    (path replay) or an albedo grid (recorded primal pass), the surface
+    or a homogeneous medium's sigma_t value (path replay) or an albedo grid or
+    value (recorded primal pass), the surface
     integrator (0) for a bitmap reflectance (recorded primal pass), or
     volpathsimple's differential-ratio-tracking adjoint (2) of a density."""
     if params is None:

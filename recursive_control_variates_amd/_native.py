@@ -24,6 +24,7 @@ STATE_OLD = 0
 STATE_NEW = 1
 PARAM_TEXTURE = 0
 PARAM_GRID = 1
+PARAM_VALUE = 2
 BSDF_DIFFUSE = 0
 BSDF_NULL = 1
 EMITTER_AREA = 0
@@ -61,7 +62,7 @@ class MediumDesc(C.Structure):
     _fields_ = [("sigma_t_param", C.c_int32), ("albedo_param", C.c_int32),
                 ("albedo_rgb", C.c_float * 3), ("scale", C.c_float),
                 ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
-                ("sample_emitters", C.c_int32), ("majorant_factor", C.c_int32)]
+                ("sample_emitters", C.c_int32), ("majorant_factor", C.c_int32), ("homogeneous", C.c_int32)]
 
 
 class CVState(C.Structure):

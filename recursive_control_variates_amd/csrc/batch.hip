@@ -327,7 +327,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
         if (any_nonzero(dL) && INTEG == RCV_INTEGRATOR_VOLPATH) {
           volpath_adjoint_samples<F>(S, J.integ, sen, seed, px, py, pixel, J.spp_g,
                                      sample_end(chunk - 1, J.spp_g, J.cl2_g), sample_end(chunk, J.spp_g, J.cl2_g), dL,
-                                     &J.gt, stack, cnt);
+                                     &J.gt, stack, cnt, nullptr);
         } else if (any_nonzero(dL)) {
           const int s_end = sample_end(chunk, J.spp_g, J.cl2_g);
           for (int s = sample_end(chunk - 1, J.spp_g, J.cl2_g); s < s_end; ++s) {
@@ -533,6 +533,7 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
       for (int k = 0; k < 3; ++k) J.sg_res[k] = m.sg_res[k];
     }
   RCV_REQUIRE(n_sg <= 1, "rcv_batch_add_job: the optimised grid drives %d media with supergrids (at most 1)", n_sg);
+  RCV_REQUIRE(!(s->features() & kFeatHomog), "rcv_batch_add_job: batched jobs take heterogeneous media only");
   if (J.itype == RCV_INTEGRATOR_VOLPATH) b->feat |= s->features();
   b->host_jobs.push_back(J);
   b->scenes.push_back(s);

@@ -136,19 +136,21 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
   const int64_t pixel = tid >> chunk_log2;
   const int chunk = int(tid & ((1 << chunk_log2) - 1));
   LocalCounters cnt{S.counters};
+  HomSum hom{0.0f, v3(0.f)};
   if (pixel < n_pix) {
     const int px = int(pixel % film_w), py = int(pixel / film_w);
     const float inv = float(spp);
     const V3 dL = v3(grad_img[3 * pixel] / inv, grad_img[3 * pixel + 1] / inv, grad_img[3 * pixel + 2] / inv);
     if (any_nonzero(dL) && INTEG == RCV_INTEGRATOR_VOLPATH) {
       volpath_adjoint_samples<F>(S, I, sen, seed, px, py, pixel, spp, sample_end(chunk - 1, spp, chunk_log2),
-                                 sample_end(chunk, spp, chunk_log2), dL, &G, stack, cnt);
+                                 sample_end(chunk, spp, chunk_log2), dL, &G, stack, cnt, &hom);
     } else if (any_nonzero(dL)) {
       const int s_end = sample_end(chunk, spp, chunk_log2);
       for (int s = sample_end(chunk - 1, spp, chunk_log2); s < s_end; ++s)
         adjoint_one_sample<INTEG, F>(S, I, sen, seed, px, py, pixel, spp, s, dL, G, stack, cnt);
     }
   }
+  if constexpr (INTEG == RCV_INTEGRATOR_VOLPATH && (F & kFeatHomog) != 0) flush_homogeneous(S, G, hom);
 }
 
 // Path-replay adjoint of the volume path tracer as a persistent launch: one
@@ -169,6 +171,7 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
   LocalCounters cnt{S.counters};
   VolPath<1, true, F> t;
   t.L[0] = v3(0.f);
+  t.init_hom();
   bool live = false, want_replay = false;
   int s = 0, s_end = 0, px = 0, py = 0;
   int64_t pixel = 0;
@@ -207,6 +210,7 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
       if (!want_replay) ++s;
     }
   }
+  if constexpr ((F & kFeatHomog) != 0) flush_homogeneous(S, G, t.hom);
 }
 
 // The per-sample integrators' adjoint (volpathsimple, surface path tracer) as
@@ -457,6 +461,8 @@ static int check_simple(const Scene *s, const char *fn) {
     return fail(RCV_E_SCENE, "%s: volpathsimple: Expected a single shape with an interior medium.", fn);
   if (s->features() & kFeatSurfaces)
     return fail(RCV_E_SCENE, "%s: volpathsimple: the medium boundary must use a null BSDF", fn);
+  if (s->features() & kFeatHomog)
+    return fail(RCV_E_SCENE, "%s: volpathsimple: heterogeneous media only", fn);
   if (s->emitters.size() != 1 || s->emitters[0].type != RCV_EMITTER_CONSTANT)
     return fail(RCV_E_SCENE, "%s: volpathsimple: the only emitter must be an infinite light", fn);
   return RCV_OK;
@@ -518,19 +524,19 @@ int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint
                          ucl2, s->partial, s->work_queue, units);
     };
     if (ns == 2) {
-      with_features(s->features(), [&](auto f) { go(render_persistent_kernel<2, decltype(f)::value>); });
+      with_render_features(s->features(), [&](auto f) { go(render_persistent_kernel<2, decltype(f)::value>); });
       hipLaunchKernelGGL(reduce_units_kernel<2>, dim3(ceil_div(n_pix, 256)), dim3(256), 0, stream, s->partial, n_pix,
                          ucl2, spp, img_old, img_new);
     } else {
-      with_features(s->features(), [&](auto f) { go(render_persistent_kernel<1, decltype(f)::value>); });
+      with_render_features(s->features(), [&](auto f) { go(render_persistent_kernel<1, decltype(f)::value>); });
       hipLaunchKernelGGL(reduce_units_kernel<1>, dim3(ceil_div(n_pix, 256)), dim3(256), 0, stream, s->partial, n_pix,
                          ucl2, spp, img_old, img_new);
     }
   } else if (d->type == RCV_INTEGRATOR_VOLPATH) {
     if (ns == 2)
-      with_features(s->features(), [&](auto f) { launch_render_kernel<2, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>(L); });
+      with_render_features(s->features(), [&](auto f) { launch_render_kernel<2, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>(L); });
     else
-      with_features(s->features(), [&](auto f) { launch_render_kernel<1, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>(L); });
+      with_render_features(s->features(), [&](auto f) { launch_render_kernel<1, RCV_INTEGRATOR_VOLPATH, decltype(f)::value>(L); });
   } else {
     if (ns == 2)
       launch_render_kernel<2, RCV_INTEGRATOR_PATH, 0>(L);
@@ -593,7 +599,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
                          G);
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH && !d->samples_per_thread)
-    with_features(s->features(), [&](auto f) {
+    with_render_features(s->features(), [&](auto f) {
       // the waves that fit, pulling units of >= kAdjUnitSamples samples
       // (fewer where the image is too small to fill the device otherwise)
       constexpr int F = decltype(f)::value;
@@ -607,7 +613,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
                          I, sensor, seed, spp, ucl2, grad_img, G, s->work_queue, units);
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH)
-    with_features(s->features(), [&](auto f) {
+    with_render_features(s->features(), [&](auto f) {
       hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), dim3(grid), dim3(block),
                          stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);
     });

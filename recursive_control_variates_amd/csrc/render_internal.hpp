@@ -27,6 +27,22 @@ inline void with_features(int f, Fn &&fn) {
   }
 }
 
+// with_features for the single-scene volume kernels, which also take scenes
+// with a homogeneous medium: those run one of two kernels, with surface code
+// (and heterogeneous media without albedo grids or supergrids), or with every
+// feature.
+template <typename Fn>
+inline void with_render_features(int f, Fn &&fn) {
+  if (f & kFeatHomog) {
+    if (f & (kFeatAlbedoGrid | kFeatSupergrid))
+      fn(std::integral_constant<int, kFeatAll | kFeatHomog>{});
+    else
+      fn(std::integral_constant<int, kFeatSurfaces | kFeatHomog>{});
+    return;
+  }
+  with_features(f, fn);
+}
+
 // Waves per SIMD of the render kernels: the trackers are bound by memory
 // latency, so occupancy pays while spills stay small. Measured: the volume
 // path tracer (<= 128 VGPRs, few spills) is fastest at four; volpathsimple and

@@ -405,6 +405,17 @@ int Scene::upload(hipStream_t stream) {
   for (int i = 0; i < int(emitters.size()); ++i)
     if (emitters[i].type == RCV_EMITTER_CONSTANT) v.env_emitter = i;
   v.n_media = int(media.size());
+  v.handle_null = 0;
+  v.nee_homog = 0;
+  v.hom_medium = -1;
+  for (int i = 0; i < int(media.size()); ++i) {
+    if (media[i].homogeneous) {
+      v.nee_homog = 1;
+      v.hom_medium = i;
+    } else {
+      v.handle_null = 1;
+    }
+  }
   for (int k = 0; k < 3; ++k) v.bs_center[k] = bs_center[k];
   v.bs_radius = bs_radius;
   for (int i = 0; i < kMaxParams; ++i) {
@@ -514,6 +525,10 @@ int Scene::features() const {
   for (const ShapeInfo &sh : shapes)
     if (sh.bsdf < 0 || bsdfs[sh.bsdf].type != RCV_BSDF_NULL) f |= kFeatSurfaces;
   for (const MediumInfo &m : media) {
+    if (m.homogeneous) {
+      f |= kFeatHomog;
+      continue;
+    }
     if (m.albedo_param >= 0) f |= kFeatAlbedoGrid;
     if (m.supergrid_offset >= 0) f |= kFeatSupergrid;
   }
@@ -557,9 +572,11 @@ extern "C" int rcv_scene_destroy(rcv_scene *scene) {
 extern "C" int rcv_scene_add_param(rcv_scene *s, int32_t kind, int32_t nx, int32_t ny, int32_t nz,
                                    int32_t channels, int32_t *out_id) {
   REQUIRE_OPEN(s, "rcv_scene_add_param");
-  RCV_REQUIRE(kind == RCV_PARAM_TEXTURE || kind == RCV_PARAM_GRID, "rcv_scene_add_param: bad kind %d", kind);
+  RCV_REQUIRE(kind == RCV_PARAM_TEXTURE || kind == RCV_PARAM_GRID || kind == RCV_PARAM_VALUE,
+              "rcv_scene_add_param: bad kind %d", kind);
   RCV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && channels > 0, "rcv_scene_add_param: bad shape");
   RCV_REQUIRE(kind == RCV_PARAM_GRID || nz == 1, "rcv_scene_add_param: textures have nz == 1");
+  RCV_REQUIRE(kind != RCV_PARAM_VALUE || (nx == 1 && ny == 1), "rcv_scene_add_param: a value has nx = ny = nz = 1");
   RCV_REQUIRE(channels == 1 || channels == 3, "rcv_scene_add_param: channels must be 1 or 3");
   RCV_REQUIRE(int(s->params.size()) < kMaxParams, "rcv_scene_add_param: at most %d parameters", kMaxParams);
   Param p;
@@ -593,6 +610,7 @@ extern "C" int rcv_scene_resize_param(rcv_scene *s, int32_t param, int32_t nx, i
   Param &p = s->params[param];
   RCV_REQUIRE(nx > 0 && ny > 0 && nz > 0, "rcv_scene_resize_param: sizes must be positive");
   RCV_REQUIRE(p.kind != RCV_PARAM_TEXTURE || nz == 1, "rcv_scene_resize_param: a texture has nz = 1");
+  RCV_REQUIRE(p.kind != RCV_PARAM_VALUE, "rcv_scene_resize_param: a value parameter has no resolution");
   RCV_REQUIRE(int64_t(nx) * ny * nz * p.ch < (int64_t(1) << 31), "rcv_scene_resize_param: too many elements");
   p.nx = nx;
   p.ny = ny;
@@ -612,6 +630,8 @@ extern "C" int rcv_scene_set_majorant_factor(rcv_scene *s, int32_t medium, int32
               medium);
   RCV_REQUIRE(factor >= 0, "rcv_scene_set_majorant_factor: factor must be >= 0");
   if (s->batch_refs > 0) return fail(RCV_E_STATE, "rcv_scene_set_majorant_factor: the scene belongs to a batch");
+  RCV_REQUIRE(!s->media[medium].homogeneous || factor == 0,
+              "rcv_scene_set_majorant_factor: a homogeneous medium has no majorant supergrid");
   s->media[medium].majorant_factor = factor;
   return s->relayout_supergrids(as_stream(stream));
 }
@@ -637,17 +657,26 @@ extern "C" int rcv_scene_add_bsdf(rcv_scene *s, int32_t type, const float rgb[3]
 extern "C" int rcv_scene_add_medium(rcv_scene *s, const rcv_medium_desc *d, int32_t *out_id) {
   REQUIRE_OPEN(s, "rcv_scene_add_medium");
   RCV_REQUIRE(d, "rcv_scene_add_medium: null desc");
+  const bool hom = d->homogeneous != 0;
+  const int pk = hom ? RCV_PARAM_VALUE : RCV_PARAM_GRID;
   RCV_REQUIRE(d->sigma_t_param >= 0 && d->sigma_t_param < int(s->params.size()) &&
-                  s->params[d->sigma_t_param].kind == RCV_PARAM_GRID && s->params[d->sigma_t_param].ch == 1,
-              "rcv_scene_add_medium: sigma_t must be a 1-channel grid parameter");
+                  s->params[d->sigma_t_param].kind == pk && s->params[d->sigma_t_param].ch == 1,
+              "rcv_scene_add_medium: sigma_t must be a 1-channel %s parameter", hom ? "value" : "grid");
   if (d->albedo_param >= 0)
-    RCV_REQUIRE(d->albedo_param < int(s->params.size()) && s->params[d->albedo_param].kind == RCV_PARAM_GRID &&
+    RCV_REQUIRE(d->albedo_param < int(s->params.size()) && s->params[d->albedo_param].kind == pk &&
                     s->params[d->albedo_param].ch == 3,
-                "rcv_scene_add_medium: albedo must be a 3-channel grid parameter");
+                "rcv_scene_add_medium: albedo must be a 3-channel %s parameter", hom ? "value" : "grid");
   RCV_REQUIRE(d->majorant_factor >= 0, "rcv_scene_add_medium: majorant_factor must be >= 0");
-  for (int k = 0; k < 3; ++k)
-    RCV_REQUIRE(d->bbox_max[k] > d->bbox_min[k], "rcv_scene_add_medium: empty bbox");
+  if (hom) {
+    RCV_REQUIRE(d->majorant_factor == 0, "rcv_scene_add_medium: a homogeneous medium has no majorant supergrid");
+    for (const MediumInfo &o : s->media)
+      RCV_REQUIRE(!o.homogeneous, "rcv_scene_add_medium: at most one homogeneous medium per scene");
+  } else {
+    for (int k = 0; k < 3; ++k)
+      RCV_REQUIRE(d->bbox_max[k] > d->bbox_min[k], "rcv_scene_add_medium: empty bbox");
+  }
   MediumInfo m{};
+  m.homogeneous = hom ? 1 : 0;
   m.sigma_param = d->sigma_t_param;
   m.albedo_param = d->albedo_param;
   for (int k = 0; k < 3; ++k) {

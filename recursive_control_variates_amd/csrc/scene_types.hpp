@@ -73,8 +73,8 @@ struct EmitterInfo {
 };
 
 struct MediumInfo {
-  int32_t sigma_param;
-  int32_t albedo_param;  // -1: constant albedo
+  int32_t sigma_param;   // grid (heterogeneous) or 1-value parameter (homogeneous)
+  int32_t albedo_param;  // -1: constant albedo; a 3-channel grid or 3-value parameter
   float albedo[3];
   float scale;
   float lo[3];
@@ -83,6 +83,7 @@ struct MediumInfo {
   int32_t majorant_factor;
   int32_t supergrid_offset;  // float offset of this medium's supergrids (2 states) or -1
   int32_t sg_res[3];         // supergrid resolution
+  int32_t homogeneous;       // 1: constant sigma_t and albedo, unbounded (lo/hi unused)
 };
 
 struct ViewInfo {
@@ -106,6 +107,7 @@ constexpr int kFeatSurfaces = 1;    // a shape whose BSDF is not null (diffuse s
 constexpr int kFeatAlbedoGrid = 2;  // a medium whose albedo is a grid
 constexpr int kFeatSupergrid = 4;   // a medium with a majorant supergrid
 constexpr int kFeatAll = 7;
+constexpr int kFeatHomog = 8;       // a homogeneous medium
 
 struct ParamBind {
   const float *data[2];  // RCV_STATE_OLD, RCV_STATE_NEW
@@ -136,6 +138,13 @@ struct SceneView {
   int32_t n_emitters;
   int32_t env_emitter;  // -1 if none
   int32_t n_media;
+  // Integrator flags of upstream prbvolpath's prepare_scene, reached from
+  // plugins/twostatevolpath.py:99: null scattering (delta tracking) when any
+  // medium is heterogeneous, otherwise free-flight sampling weighted by
+  // transmittance / pdf; analytic transmittance of homogeneous media in NEE.
+  int32_t handle_null;
+  int32_t nee_homog;
+  int32_t hom_medium;   // the scene's homogeneous medium (at most one) or -1
   float bs_center[3];
   float bs_radius;
   ParamBind params[kMaxParams];

@@ -195,19 +195,52 @@ __device__ inline float supergrid_free_flight(const SceneView &S, const MediumIn
   return ts;
 }
 
+// Optical depth of a free flight, -log(1 - u) (plugins/twostatemedium.py:24).
+__device__ inline float tau_of(float u) { return -logf(1.0f - u); }
+
 // TwoStateMedium.sample_interaction_twostates (plugins/twostatemedium.py:
 // 15-107): one global majorant max(maj_old, maj_new) (:71-78), or the
 // supergrid DDA when the medium has a majorant supergrid (:25-70, 82-85).
 template <int NS, int F>
 __device__ inline void sample_medium(const SceneView &S, int mid, const Ray &ray, float u, MediumEv<NS> &ev) {
   const MediumInfo &M = S.media[mid];
+  if ((F & kFeatHomog) && M.homogeneous) {
+    // A homogeneous medium is unbounded (its interaction interval is [0,
+    // ray.maxt]); its majorant is sigma_t itself, combined over the states
+    // (:71-78), and its coefficients are constants.
+    const ParamBind &P = S.params[M.sigma_param];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const int s = state_index<NS>(k);
+      ev.st[k] = M.scale * P.data[s][0];
+      V3 alb;
+      if (M.albedo_param >= 0) {
+        const float *a = S.params[M.albedo_param].data[s];
+        alb = v3(a[0], a[1], a[2]);
+      } else {
+        alb = v3(M.albedo[0], M.albedo[1], M.albedo[2]);
+      }
+      if (k == NS - 1) ev.alb = alb;
+      ev.ss[k] = alb * ev.st[k];
+    }
+    const float maj = NS == 2 ? fmaxf(ev.st[0], ev.st[1]) : ev.st[0];
+    const float t = tau_of(u) / maj;
+    ev.valid = maj > 0.0f && maj < kInf && t <= ray.maxt;
+    ev.mint = 0.0f;
+    ev.maj = maj;
+    ev.t = ev.valid ? t : kInf;
+    ev.p = ray_at(ray, t);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) ev.sn[k] = maj - ev.st[k];
+    return;
+  }
   float t0, t1;
   bool ok = ray_aabb(ray, M.lo, M.hi, t0, t1);
   ok = ok && (t0 < kInf || t1 < kInf);
   const float mint = ok ? fmaxf(0.0f, t0) : 0.0f;
   const float maxt = ok ? fminf(ray.maxt, t1) : kInf;
   const float scale = M.scale;
-  const float tau = -logf(1.0f - u);
+  const float tau = tau_of(u);
   float maj = 0.0f, t = kInf;
   if ((F & kFeatSupergrid) && M.supergrid_offset >= 0) {
     if (ok) t = supergrid_free_flight<NS>(S, M, ray, mint, maxt, tau, maj);
@@ -288,6 +321,7 @@ __device__ inline bool null_bsdf(const SceneView &S, int shape) {
 constexpr int kNeeRecords = 32;
 struct NeeRecord {
   int n;
+  float hseg;  // length of the shadow ray inside the homogeneous medium
   float4 e[kNeeRecords];
   int med[kNeeRecords];
 };
@@ -296,11 +330,17 @@ struct NeeRecord {
 // direction without visibility, then ratio tracking of both transmittances.
 // GRAD = true replays the same random numbers and scatters
 // d(log T)/d(density) * gscale into the density gradient.
+//
+// In a homogeneous medium (:422-431) the shadow ray advances straight to the
+// next surface or the end of the segment with the analytic transmittance
+// exp(-sigma_t d) of each state (TwoStateMedium.eval_tr_old/new,
+// plugins/twostatemedium.py:109-115); GRAD adds d(log T)/d(value) * gscale =
+// -scale d gscale to *hom_acc.
 template <int NS, bool GRAD, int F>
 __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, bool ref_surface, int medium,
                                      Pcg32 &rng, V3 em[NS], DirSample &ds, const LdsStack &stack,
                                      LocalCounters &cnt, float gscale, const GradTargets *G,
-                                     NeeRecord *rec = nullptr) {
+                                     NeeRecord *rec = nullptr, float *hom_acc = nullptr) {
   const float u1 = rng.next_1d();
   const float u2 = rng.next_1d();
   const V3 ev = sample_emitter_direction(S, ref_p, u1, u2, ds);
@@ -334,6 +374,9 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
     MediumEv<NS> mev;
     mev.valid = false;
     mev.t = kInf;
+    float trm[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) trm[k] = 1.0f;
     if (active_medium) {
       const float u = rng.next_1d();
       sample_medium<NS, F>(S, medium, ray, u, mev);
@@ -342,12 +385,18 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
         mev.valid = false;
         mev.t = kInf;
       }
+      if ((F & kFeatHomog) && S.media[medium].homogeneous) {
+        const float seg = fminf(fminf(remaining, si.t), si.t) - mev.mint;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) trm[k] = expf(-(seg * mev.st[k]));
+        mev.valid = false;
+        mev.t = kInf;
+        if (rec) rec->hseg += seg;
+        if (GRAD && hom_acc && G->g[S.media[medium].sigma_param]) *hom_acc += -gscale * S.media[medium].scale * seg;
+      }
     }
     const bool escaped = active_medium && !mev.valid;
     active_medium = active_medium && mev.valid;
-    float trm[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) trm[k] = 1.0f;
     if (active_medium) {
       ray.o = mev.p;
       si.t = si.t - mev.t;
@@ -432,6 +481,35 @@ struct NoTape {};
 template <int F>
 constexpr bool kTapeAdjoint = (F & kFeatSupergrid) && !(F & kFeatAlbedoGrid);
 
+// Adjoint of the homogeneous medium's constants (sigma_t value, albedo):
+// every event adds to per-lane sums, which a kernel reduces over its
+// wavefront and adds to the gradient once, at its end (flush_homogeneous).
+struct HomSum {
+  float s;  // d/d(sigma_t value)
+  V3 a;     // d/d(albedo)
+};
+struct NoHomSum {};
+
+// All 64 lanes of the wavefront must be active.
+__device__ inline void flush_homogeneous(const SceneView &S, const GradTargets &G, HomSum h) {
+  for (int off = 32; off > 0; off >>= 1) {
+    h.s += __shfl_xor(h.s, off, 64);
+    h.a.x += __shfl_xor(h.a.x, off, 64);
+    h.a.y += __shfl_xor(h.a.y, off, 64);
+    h.a.z += __shfl_xor(h.a.z, off, 64);
+  }
+  if (__lane_id() == 0 && S.hom_medium >= 0) {
+    const MediumInfo &M = S.media[S.hom_medium];
+    if (float *g = G.g[M.sigma_param]) atomicAdd(g, h.s);
+    if (M.albedo_param >= 0)
+      if (float *g = G.g[M.albedo_param]) {
+        atomicAdd(g + 0, h.a.x);
+        atomicAdd(g + 1, h.a.y);
+        atomicAdd(g + 2, h.a.z);
+      }
+  }
+}
+
 // One camera sample of the two-state volumetric path tracer, advanced one
 // bounce per call (the body of the reference's loop, :175-380), so that a
 // kernel can start a lane's next camera sample as soon as its path ends
@@ -455,6 +533,15 @@ struct VolPath {
   bool needs_int, specular_chain, replay, nee_grad;
   AdjState adj;
   std::conditional_t<ADJ && kTapeAdjoint<F>, PathTape, NoTape> tape;
+  std::conditional_t<ADJ && (F & kFeatHomog) != 0, HomSum, NoHomSum> hom;
+
+  // Zeroes the homogeneous-medium sums (once per lane, before its samples).
+  __device__ void init_hom() {
+    if constexpr (ADJ && (F & kFeatHomog)) {
+      hom.s = 0.0f;
+      hom.a = v3(0.f);
+    }
+  }
 
   __device__ void begin(const Pcg32 &r, const Ray &camera, LocalCounters &cnt, bool is_replay) {
     rng = r;
@@ -548,6 +635,10 @@ struct VolPath {
 
     bool active_medium = medium >= 0;
     bool active_surface = !active_medium;
+    const bool in_hom = (F & kFeatHomog) && active_medium && S.media[medium].homogeneous;
+    // delta tracking with null collisions unless no medium of the scene is
+    // heterogeneous (handle_null_scattering of upstream prbvolpath)
+    const bool null_mode = !(F & kFeatHomog) || S.handle_null;
 
     // ---- free flight with the combined majorant (:193-206)
     MediumEv<NS> mev;
@@ -557,6 +648,8 @@ struct VolPath {
       const float u = rng.next_1d();
       sample_medium<NS, F>(S, medium, ray, u, mev);
       cnt.add(kCntCollisions, 1);
+      // a homogeneous medium's collision bounds the surface search (:207)
+      if (in_hom && mev.valid) ray.maxt = mev.t;
       if (needs_int) {
         Hit h;
         trace<false>(S, ray, h, stack);
@@ -572,12 +665,30 @@ struct VolPath {
     const bool escaped = active_medium && !mev.valid;
     active_medium = active_medium && mev.valid;
 
-    // ---- null / real decision with the averaged probability (:219-231)
     bool act_null = false, act_scatter = false;
     V3 w[NS];
   #pragma unroll
     for (int k = 0; k < NS; ++k) w[k] = v3(1.f);
-    if (active_medium) {
+    // ---- without null scattering: transmittance over the NEW state's
+    // free-flight pdf (:212-213, 232-247)
+    float ff_t = 0.0f;
+    if ((F & kFeatHomog) && !null_mode && (escaped || active_medium)) {
+      ff_t = fminf(mev.t, si.t) - mev.mint;
+      const float sig = mev.st[NS - 1];
+      const float tr = expf(-(ff_t * sig));
+      const float pdf = si.t < mev.t ? tr : tr * sig;
+  #pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const float r = pdf > 0.0f ? expf(-(ff_t * mev.st[k])) / pdf : 0.0f;
+        if (escaped)
+          thr[k] = thr[k] * r;
+        else
+          w[k] = w[k] * r;
+      }
+      act_scatter = active_medium;
+    }
+    // ---- null / real decision with the averaged probability (:219-231)
+    if (active_medium && null_mode) {
       const float maj = mev.maj;
       float p;
       if (NS == 2)
@@ -606,7 +717,10 @@ struct VolPath {
     }
     if (act_scatter) {
       cnt.add(kCntRealScatter, 1);
-      if (NS == 2) {
+      if (!null_mode) {
+  #pragma unroll
+        for (int k = 0; k < NS; ++k) w[k] = w[k] * mev.ss[k];  // (:261-263)
+      } else if (NS == 2) {
         const float den = mev.st[1] + mev.st[0];
         w[1] = w[1] * (2.0f * mev.ss[1] / den);
         w[0] = w[0] * (2.0f * mev.ss[0] / den);
@@ -633,7 +747,33 @@ struct VolPath {
         }
       }
     }
-    if (ADJ && replay && (act_null || act_scatter)) {
+    if constexpr (ADJ && (F & kFeatHomog) != 0) {
+      if (replay && in_hom && (escaped || active_medium)) {
+        // d log(weight)/d(value) (sigma_t = scale value) of this segment's
+        // weight times the radiance it multiplies: null collision -scale /
+        // sigma_n, real collision +scale / sigma_t; without null scattering
+        // exp(-sigma_t t) / pdf gives -scale t, and a real collision's
+        // sigma_s (= albedo sigma_t) +scale / sigma_t.
+        const MediumInfo &M = S.media[medium];
+        const float lr = dot(adj.dL, adj.Lrem);
+        float coef = 0.0f;
+        if (null_mode) {
+          if (act_null)
+            coef = mev.sn[0] != 0.0f ? -M.scale / mev.sn[0] : 0.0f;
+          else if (act_scatter)
+            coef = M.scale / mev.st[0];
+        } else {
+          coef = -ff_t * M.scale + (act_scatter ? M.scale / mev.st[0] : 0.0f);
+        }
+        hom.s += lr * coef;
+        if (act_scatter && M.albedo_param >= 0) {
+          const V3 dl = adj.dL * adj.Lrem;
+          hom.a += v3(mev.alb.x != 0.f ? dl.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? dl.y / mev.alb.y : 0.f,
+                      mev.alb.z != 0.f ? dl.z / mev.alb.z : 0.f);
+        }
+      }
+    }
+    if (ADJ && replay && !in_hom && (act_null || act_scatter)) {
       // d log(weight) / d theta at this collision times the radiance that
       // the weight multiplies (everything gathered after this point).
       const MediumInfo &M = S.media[medium];
@@ -722,6 +862,7 @@ struct VolPath {
       const Pcg32 rng_nee = rng;
       NeeRecord rec;
       rec.n = 0;
+      rec.hseg = 0.0f;
       nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G,
                                   ADJ && nee_grad ? &rec : nullptr);
       V3 nw[NS];
@@ -748,6 +889,9 @@ struct VolPath {
         const float cs = dot(adj.dL, c[0]);
         if (cs != 0.0f && any_nonzero(c[0])) {
           if (rec.n <= kNeeRecords) {
+            if constexpr (ADJ && (F & kFeatHomog) != 0) {
+              if (rec.hseg != 0.0f) hom.s += -cs * S.media[S.hom_medium].scale * rec.hseg;
+            }
             // d(log T)/d(density) at each recorded shadow-ray collision
             for (int j = 0; j < rec.n; ++j) {
               const MediumInfo &M = S.media[rec.med[j]];
@@ -764,7 +908,10 @@ struct VolPath {
             Pcg32 r2 = rng_nee;
             V3 em2[NS];
             DirSample ds2;
-            if constexpr (ADJ)
+            if constexpr (ADJ && (F & kFeatHomog) != 0)
+              nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs,
+                                         G, nullptr, &hom.s);
+            else if constexpr (ADJ)
               nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs,
                                          G);
           }
@@ -862,9 +1009,11 @@ __device__ void volpath_render_samples(const SceneView &S, const IntegArgs &I, c
 template <int F>
 __device__ void volpath_adjoint_samples(const SceneView &S, const IntegArgs &I, const SensorInfo &sen, uint32_t seed,
                                         int px, int py, int64_t pixel, int spp, int s0, int s1, V3 dL,
-                                        const GradTargets *G, const LdsStack &stack, LocalCounters &cnt) {
+                                        const GradTargets *G, const LdsStack &stack, LocalCounters &cnt,
+                                        HomSum *hom_acc) {
   VolPath<1, true, F> t;
   t.L[0] = v3(0.f);  // read (unused) when the first sample starts
+  t.init_hom();
   bool live = false, want_replay = false;
   int s = s0;
   while (true) {
@@ -879,6 +1028,10 @@ __device__ void volpath_adjoint_samples(const SceneView &S, const IntegArgs &I, 
       want_replay = !t.end_adjoint(S, cnt, G);
       if (!want_replay) ++s;
     }
+  }
+  if constexpr ((F & kFeatHomog) != 0) {
+    hom_acc->s += t.hom.s;
+    hom_acc->a += t.hom.a;
   }
 }
 

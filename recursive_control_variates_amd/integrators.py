@@ -33,10 +33,50 @@ def _props(props) -> dict:
     return dict(props or {})
 
 
+class _LaunchEvents:
+    """Records a (start, end) pair of timing events on the launch stream
+    around each native launch of one kind, for benchmarks that time kernels
+    inside a running optimisation."""
+
+    def __init__(self):
+        self.pairs = []
+
+    def around(self, fn):
+        import torch
+        s = torch.cuda.current_stream()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        fn()
+        b.record(s)
+        self.pairs.append((a, b))
+
+    def elapsed_ms(self):
+        return [a.elapsed_time(b) for a, b in self.pairs]
+
+
+def _launch(events, fn):
+    if events is None:
+        fn()
+    else:
+        events.around(fn)
+
+
 class RenderIntegrator:
     """Common native render / adjoint surface."""
 
     INTEGRATOR_TYPE = N.INTEGRATOR_PATH
+    # Optional timing hooks: a _LaunchEvents per launch kind (None: off).
+    adjoint_events = None
+    twostate_events = None
+
+    def time_launches(self, on: bool = True):
+        """Start (or stop) recording timing events around every adjoint and
+        two-state launch of this integrator; returns (adjoint, two-state)
+        recorders."""
+        self.adjoint_events = _LaunchEvents() if on else None
+        self.twostate_events = _LaunchEvents() if on else None
+        return self.adjoint_events, self.twostate_events
 
     def __init__(self, props=None):
         p = _props(props)
@@ -99,9 +139,9 @@ class RenderIntegrator:
         ids = N.i32_array([scene.params[k].index for k in keys] or [0])
         ptrs = (C.c_void_p * max(1, len(keys)))(*[grads[k].data_ptr() for k in keys])
         d = self.desc()
-        N.call("rcv_render_backward", scene.handle, C.byref(d), sensor, C.c_uint32(seed & 0xFFFFFFFF), spp,
-               C.c_void_p(grad_in.data_ptr()), len(keys), ids, ptrs,
-               C.c_void_p(N.stream_handle(scene.device)))
+        _launch(self.adjoint_events, lambda: N.call(
+            "rcv_render_backward", scene.handle, C.byref(d), sensor, C.c_uint32(seed & 0xFFFFFFFF), spp,
+            C.c_void_p(grad_in.data_ptr()), len(keys), ids, ptrs, C.c_void_p(N.stream_handle(scene.device))))
         return grads
 
     def render_backward(self, scene, params, grad_in, sensor: int = 0, seed: int = 0, spp: int = 0) -> None:
@@ -178,9 +218,10 @@ class _TwoStateMixin:
         img_old = torch.empty(shape, dtype=torch.float32, device=scene.device)
         img_new = torch.empty(shape, dtype=torch.float32, device=scene.device)
         d = self.desc()
-        N.call("rcv_render_twostates", scene.handle, C.byref(d), sensor, C.c_uint32(seed & 0xFFFFFFFF), spp,
-               C.c_void_p(img_old.data_ptr()), C.c_void_p(img_new.data_ptr()),
-               C.c_void_p(N.stream_handle(scene.device)))
+        _launch(self.twostate_events, lambda: N.call(
+            "rcv_render_twostates", scene.handle, C.byref(d), sensor, C.c_uint32(seed & 0xFFFFFFFF), spp,
+            C.c_void_p(img_old.data_ptr()), C.c_void_p(img_new.data_ptr()),
+            C.c_void_p(N.stream_handle(scene.device))))
         return self._post(img_old), self._post(img_new)
 
 

@@ -70,7 +70,10 @@ class Scene:
             stream = N.stream_handle(self.device)
             for i, p in enumerate(d.params):
                 out = C.c_int32()
-                if p.kind == N.PARAM_TEXTURE:
+                if p.kind == N.PARAM_VALUE:
+                    nz = ny = nx = 1
+                    ch = p.shape[0]
+                elif p.kind == N.PARAM_TEXTURE:
                     ny, nx, ch = p.shape
                     nz = 1
                 else:
@@ -93,6 +96,7 @@ class Scene:
                 md.bbox_max[:] = list(m.bbox_max)
                 md.sample_emitters = int(m.sample_emitters)
                 md.majorant_factor = m.majorant_factor
+                md.homogeneous = int(m.homogeneous)
                 out = C.c_int32()
                 N.call("rcv_scene_add_medium", h, C.byref(md), C.byref(out))
             for e in d.emitters:
@@ -173,6 +177,8 @@ class Scene:
             self._bind(sp)
 
     def _resize(self, sp: SceneParam, shape: tuple):
+        if sp.kind == N.PARAM_VALUE:
+            raise ValueError(f"{sp.key}: a value parameter keeps its shape {sp.shape}")
         if sp.kind == N.PARAM_TEXTURE:
             ny, nx, _ = shape
             nz = 1

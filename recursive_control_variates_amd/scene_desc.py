@@ -15,7 +15,10 @@ build implements:
   ``data`` or a Mitsuba ``.vol`` ``filename``, placed by ``to_world`` or,
   with ``use_grid_bbox``, by the file's box) and an albedo (float / rgb /
   3-channel ``gridvolume``), ``scale``,
-  ``sample_emitters``, ``majorant_resolution_factor``; and the
+  ``sample_emitters``, ``majorant_resolution_factor``; ``homogeneous`` with
+  a grey ``sigma_t`` (float or ``spectrum``/``rgb`` of one value) and an
+  albedo (float / ``rgb`` / ``spectrum``), ``scale``, ``sample_emitters``
+  (tutorial.ipynb:77-88), at most one per scene; and the
   ``twostatemedium`` adapter (plugins/twostatemedium.py:5-200);
 * emitters: ``area`` (on a shape), ``constant`` (environment);
 * sensors: ``perspective`` (``fov`` along x, ``to_world``) and ``batch`` of
@@ -24,7 +27,8 @@ build implements:
 
 Differentiable parameters get the reference's key names, e.g.
 ``back.bsdf.reflectance.data`` or ``object.interior_medium.sigma_t.data``
-(experiments.py:104,546); the two-state adapters expose the ``incoming``
+(experiments.py:104,546), ``bunny.interior_medium.sigma_t.value.value``
+for a homogeneous medium (tutorial.ipynb:181); the two-state adapters expose the ``incoming``
 copy's parameters under the same keys (plugins/twostatebsdf.py:43-44,
 plugins/twostatemedium.py:186-187).
 """
@@ -39,6 +43,7 @@ from .transform import Transform4f
 
 PARAM_TEXTURE = 0
 PARAM_GRID = 1
+PARAM_VALUE = 2
 BSDF_DIFFUSE = 0
 BSDF_NULL = 1
 EMITTER_AREA = 0
@@ -52,7 +57,7 @@ SENSOR_TYPES = ("perspective", "batch")
 class ParamDesc:
     key: str
     kind: int
-    shape: tuple            # texture (H, W, C); grid (D, H, W, C)
+    shape: tuple            # texture (H, W, C); grid (D, H, W, C); value (C,)
     init: np.ndarray        # float32 initial data (incoming == new == old)
     twostate: bool
 
@@ -76,6 +81,7 @@ class MediumDesc:
     sample_emitters: bool
     majorant_factor: int
     twostate: bool
+    homogeneous: bool = False
 
 
 @dataclass
@@ -249,7 +255,11 @@ class _Loader:
     # ---- parameters
     def _add_param(self, key, kind, data, twostate):
         arr = _to_numpy(data)
-        if kind == PARAM_TEXTURE:
+        if kind == PARAM_VALUE:
+            arr = arr.reshape(-1)
+            if arr.shape[0] not in (1, 3):
+                raise ValueError(f"{key}: a value parameter has 1 or 3 channels, got {arr.shape}")
+        elif kind == PARAM_TEXTURE:
             if arr.ndim == 2:
                 arr = arr[..., None]
             if arr.ndim != 3 or arr.shape[2] not in (1, 3):
@@ -305,13 +315,15 @@ class _Loader:
             twostate = True
             d = d["incoming"]
             t = d.get("type")
-        if t != "heterogeneous":
-            raise ValueError(f"{prefix}: unsupported medium type {t!r} (heterogeneous grids only)")
+        if t not in ("heterogeneous", "homogeneous"):
+            raise ValueError(f"{prefix}: unsupported medium type {t!r} (heterogeneous, homogeneous)")
         if d.get("has_spectral_extinction", False):
             raise ValueError(f"{prefix}: spectrally varying extinction is not supported")
         phase = d.get("phase", {"type": "isotropic"})
         if phase.get("type") != "isotropic":
             raise ValueError(f"{prefix}: only the isotropic phase function is supported")
+        if t == "homogeneous":
+            return self._homogeneous(prefix, d, twostate)
         sig = d.get("sigma_t")
         if not (isinstance(sig, dict) and sig.get("type") == "gridvolume"):
             raise ValueError(f"{prefix}: sigma_t must be a gridvolume")
@@ -340,6 +352,24 @@ class _Loader:
         self.desc.media.append(MediumDesc(
             sp, ap, argb, float(d.get("scale", 1.0)), tuple(float(x) for x in lo), tuple(float(x) for x in hi),
             bool(d.get("sample_emitters", True)), int(d.get("majorant_resolution_factor", 0)), twostate))
+        return len(self.desc.media) - 1
+
+    def _homogeneous(self, prefix, d, twostate):
+        """Mitsuba's 'homogeneous' medium: constant sigma_t (grey) and albedo,
+        keys ``<prefix>.sigma_t.value.value`` and ``<prefix>.albedo.value.value``."""
+        if any(m.homogeneous for m in self.desc.media):
+            raise ValueError(f"{prefix}: at most one homogeneous medium per scene")
+        sig = _as_rgb(d.get("sigma_t", 1.0))
+        if not (sig[0] == sig[1] == sig[2]):
+            raise ValueError(f"{prefix}: sigma_t must be grey (spectrally varying extinction is not supported)")
+        sp = self._add_param(f"{prefix}.sigma_t.value.value", PARAM_VALUE, np.float32([sig[0]]), twostate)
+        alb = d.get("albedo", 0.75)
+        if isinstance(alb, dict) and alb.get("type") not in ("rgb", "spectrum"):
+            raise ValueError(f"{prefix}: a homogeneous medium's albedo is a constant (float, rgb or spectrum)")
+        ap = self._add_param(f"{prefix}.albedo.value.value", PARAM_VALUE, np.float32(_as_rgb(alb)), twostate)
+        self.desc.media.append(MediumDesc(
+            sp, ap, (0.0, 0.0, 0.0), float(d.get("scale", 1.0)), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
+            bool(d.get("sample_emitters", True)), 0, twostate, homogeneous=True))
         return len(self.desc.media) - 1
 
     # ---- shapes

@@ -119,7 +119,29 @@ def cornell_scene_dict(texture: np.ndarray, twostate: bool = True, film_res: int
     }
 
 
+def homogeneous_cornell_scene_dict(sigma_t: float = 2.5, twostate: bool = True, film_res: int = 128, spp: int = 2,
+                                   albedo=(0.85, 0.97, 0.999999), scale: float = 5.0):
+    """The tutorial's scene (tutorial.ipynb:60-95, 310-327): a Cornell box
+    (without the two inner boxes) holding an object whose interior is the
+    homogeneous medium being optimised (scale 5, sigma_t 2.5, albedo (0.85,
+    0.97, 0.999999)), wrapped in the two-state adapter for the cv integrator.
+    The tutorial's object is the Stanford bunny (a .ply file not in the tree)
+    with a rough-dielectric boundary; here it is a sphere of radius 0.45 at
+    (0, -0.55, 0) with a null boundary. Key of the optimised value:
+    ``bunny.interior_medium.sigma_t.value.value`` (tutorial.ipynb:181)."""
+    d = cornell_scene_dict(np.full((4, 4, 3), 0.725, np.float32), twostate=False, film_res=film_res, spp=spp)
+    d["back"]["bsdf"] = {"type": "diffuse", "reflectance": {"type": "rgb", "value": [0.725, 0.71, 0.68]}}
+    del d["sphere"]
+    medium = {"type": "homogeneous", "scale": scale, "sigma_t": {"type": "spectrum", "value": sigma_t},
+              "albedo": {"type": "rgb", "value": tuple(albedo)}}
+    d["bunny"] = {"type": "sphere", "center": [0.0, -0.55, 0.0], "radius": 0.45, "bsdf": {"type": "null"},
+                  "interior": ({"type": "twostatemedium", "old": medium, "new": medium, "incoming": medium}
+                               if twostate else medium)}
+    return d
+
+
 VOLUME_KEY = "object.interior_medium.sigma_t.data"
+HOMOGENEOUS_KEY = "bunny.interior_medium.sigma_t.value.value"
 ALBEDO_KEY = "object.interior_medium.albedo.data"
 TEXTURE_KEY = "back.bsdf.reflectance.data"
 

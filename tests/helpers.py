@@ -95,3 +95,33 @@ def box_chord(o, d, lo=(0, 0, 0), hi=(1, 1, 1)):
     tf = np.min(np.maximum(t0, t1), axis=1)
     tn = np.maximum(tn, 0)
     return np.where(tf > tn, tf - tn, 0.0)
+
+
+def homogeneous_box_scene(sigma_t=1.0, albedo=0.8, scale=2.0, env=1.0, w=8, h=8, spp=4, twostate=True, fov=40.0,
+                          origin=(0.5, 0.5, 2.6)):
+    """A homogeneous medium (constant sigma_t value x scale and albedo,
+    tutorial.ipynb:77-88) in a null unit cube [0,1]^3 under a constant
+    environment, one camera on +z looking at the cube centre."""
+    m = {"type": "homogeneous", "sigma_t": {"type": "spectrum", "value": sigma_t},
+         "albedo": {"type": "rgb", "value": albedo}, "scale": scale}
+    interior = {"type": "twostatemedium", "old": m, "new": m, "incoming": m} if twostate else m
+    return {
+        "type": "scene",
+        "box": {"type": "cube", "bsdf": {"type": "null"}, "to_world": T.translate([0.5, 0.5, 0.5]).scale(0.5),
+                "interior": interior},
+        "env": {"type": "constant", "radiance": {"type": "rgb", "value": env}},
+        "sensor": cam(origin, (0.5, 0.5, 0.5), fov=fov, w=w, h=h, spp=spp),
+    }
+
+
+def pixel_chords(view, w, h, n=24):
+    """Per pixel, the chords through the unit cube of an n x n grid of
+    sub-pixel camera rays, shape (h, w, n * n)."""
+    jj = (np.arange(n) + 0.5) / n
+    jx, jy = [a.ravel() for a in np.meshgrid(jj, jj)]
+    out = np.zeros((h, w, jx.size))
+    for py in range(h):
+        for px in range(w):
+            o, dd = camera_rays(view.to_world, view.fov, w, h, np.full(jx.shape, px), np.full(jx.shape, py), jx, jy)
+            out[py, px] = box_chord(o, dd)
+    return out

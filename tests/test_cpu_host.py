@@ -73,8 +73,16 @@ def test_scene_desc_errors():
     g = np.zeros((4, 4, 4, 1), np.float32)
     base = scenes.volume_scene_dict(g)
     bad = dict(base)
+    bad["object"] = dict(base["object"], interior={"type": "nanovdb"})
+    with pytest.raises(ValueError, match="heterogeneous, homogeneous"):
+        scene_desc.describe(bad)
+    bad["object"] = dict(base["object"], interior={"type": "homogeneous", "sigma_t": {"type": "rgb",
+                                                                                     "value": [1, 2, 3]}})
+    with pytest.raises(ValueError, match="grey"):
+        scene_desc.describe(bad)
+    bad["object2"] = dict(base["object"], interior={"type": "homogeneous"})
     bad["object"] = dict(base["object"], interior={"type": "homogeneous"})
-    with pytest.raises(ValueError, match="heterogeneous"):
+    with pytest.raises(ValueError, match="at most one homogeneous"):
         scene_desc.describe(bad)
     bad = dict(base)
     bad["envmap2"] = {"type": "constant"}
@@ -87,6 +95,35 @@ def test_scene_desc_errors():
         scene_desc.describe(bad)
     with pytest.raises(ValueError, match="type 'scene'"):
         scene_desc.describe({"type": "cube"})
+
+
+def test_homogeneous_medium_description():
+    # the tutorial's medium (tutorial.ipynb:77-88) inside a null-BSDF sphere,
+    # wrapped in the two-state adapter (tutorial.ipynb:310-316)
+    medium = {"type": "homogeneous", "scale": 5, "sigma_t": {"type": "spectrum", "value": 2.5},
+              "albedo": {"type": "rgb", "value": (0.85, 0.97, 0.999999)}}
+    d = scene_desc.describe({
+        "type": "scene",
+        "bunny": {"type": "sphere", "radius": 0.5, "bsdf": {"type": "null"},
+                  "interior": {"type": "twostatemedium", "old": medium, "new": medium, "incoming": medium}},
+        "env": {"type": "constant"},
+        "sensor": {"type": "perspective", "film": {"type": "hdrfilm", "width": 8, "height": 8}},
+    })
+    keys = [p.key for p in d.params]
+    assert keys == ["bunny.interior_medium.sigma_t.value.value", "bunny.interior_medium.albedo.value.value"]
+    assert [p.kind for p in d.params] == [scene_desc.PARAM_VALUE] * 2
+    assert d.params[0].shape == (1,) and d.params[1].shape == (3,)
+    np.testing.assert_array_equal(d.params[0].init, np.float32([2.5]))
+    np.testing.assert_array_equal(d.params[1].init, np.float32([0.85, 0.97, 0.999999]))
+    m = d.media[0]
+    assert m.homogeneous and m.twostate and m.scale == 5.0 and m.majorant_factor == 0
+    # a plain float sigma_t and albedo
+    d = scene_desc.describe({
+        "type": "scene", "s": {"type": "sphere", "bsdf": {"type": "null"},
+                               "interior": {"type": "homogeneous", "sigma_t": 1.5, "albedo": 0.5}},
+        "sensor": {"type": "perspective"}})
+    assert d.params[0].init[0] == np.float32(1.5) and np.all(d.params[1].init == np.float32(0.5))
+    assert not d.media[0].twostate
 
 
 def test_transform_conventions():

@@ -1,0 +1,196 @@
+"""GPU parity of the homogeneous medium (upstream prbvolpath without null
+scattering, reached through plugins/twostatevolpath.py:193-263, 422-431, and
+TwoStateMedium.eval_tr_old/new, plugins/twostatemedium.py:109-115) against
+the CPU oracle: same random numbers sample by sample, the two-state
+invariants, the closed-form absorber, the adjoint of the medium's sigma_t
+value and albedo, a scene mixing both medium kinds, and the tutorial's
+optimisation loop on the homogeneous-medium Cornell box."""
+import numpy as np
+import pytest
+import torch
+
+import render_ref as R
+from conftest import announce
+from helpers import homogeneous_box_scene, pixel_chords, volume_box_scene
+import recursive_control_variates_amd as rcv
+from recursive_control_variates_amd import scenes
+from recursive_control_variates_amd.transform import T
+
+pytestmark = pytest.mark.gpu
+VOL = 1
+SIG_KEY = "box.interior_medium.sigma_t.value.value"
+ALB_KEY = "box.interior_medium.albedo.value.value"
+
+
+def _close_fraction(a, b, rtol=1e-3, atol=1e-4):
+    return np.mean(np.abs(a - b) <= atol + rtol * np.abs(b))
+
+
+def _two_state(sc, key, old, new):
+    p = rcv.traverse(sc)
+    for v in (old, new):
+        p[key] = torch.as_tensor(np.float32(v)).reshape(-1).cuda()
+        p.update()
+    return p
+
+
+@pytest.mark.parametrize("s_old,s_new", [(0.6, 1.1), (1.1, 0.6)])
+def test_two_state_matches_oracle_sample_by_sample(cuda, s_old, s_new):
+    announce("homogeneous two-state vs oracle fp32", s_old, s_new)
+    sc = rcv.load_dict(homogeneous_box_scene(sigma_t=s_old, albedo=(0.5, 0.7, 0.9), scale=3.0, w=16, h=12))
+    _two_state(sc, SIG_KEY, s_old, s_new)
+    integ = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 3})
+    H, F = integ.render_twostates(sc, 0, 123, 8)
+    alb = np.float32([0.5, 0.7, 0.9])
+    Ho, Fo = R.render(sc.desc, VOL, 8, 3, 123, 8, params_old=[np.float32([s_old]), alb],
+                      params_new=[np.float32([s_new]), alb], precision=32)
+    for g, o in ((H, Ho), (F, Fo)):
+        g = g.cpu().numpy()
+        assert _close_fraction(g, o) > 0.98, _close_fraction(g, o)
+        assert abs(g.mean() - o.mean()) < 1e-3 * abs(o.mean()) + 1e-5
+
+
+def test_equal_states_bitwise_and_plain(cuda):
+    announce("homogeneous H == F == plain")
+    sc = rcv.load_dict(homogeneous_box_scene(sigma_t=0.8, albedo=0.7, scale=4.0, w=20, h=10))
+    ts = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 12, "rr_depth": 5})
+    plain = rcv.load_dict({"type": "prbvolpath", "max_depth": 12, "rr_depth": 5})
+    H, F = ts.render_twostates(sc, 0, 77, 8)
+    P = plain.render(sc, 0, 77, 8)
+    assert torch.equal(H, F)
+    assert torch.equal(P, F)
+
+
+def test_absorber_closed_form(cuda):
+    # albedo 0: E[L] = exp(-sigma chord) per camera ray
+    announce("homogeneous absorber closed form")
+    v, scale, w, h, spp = 0.6, 2.0, 12, 12, 1024
+    sc = rcv.load_dict(homogeneous_box_scene(sigma_t=v, albedo=0.0, scale=scale, w=w, h=h, fov=50))
+    plain = rcv.load_dict({"type": "prbvolpath", "max_depth": 16, "rr_depth": 16})
+    imgs = np.stack([plain.render(sc, 0, 500 + s, spp).cpu().numpy()[..., 0] for s in range(8)])
+    c = pixel_chords(sc.desc.sensors[0].views[0], w, h)
+    expect = np.exp(-scale * v * c).mean(-1)
+    whole = (c > 0).all(-1)
+    se = imgs.std(0, ddof=1) / np.sqrt(len(imgs)) + 1e-7
+    z = np.where(whole, (imgs.mean(0) - expect) / se, 0.0)
+    assert np.abs(z).max() < 5.5, np.abs(z).max()
+    assert abs(z[whole].mean()) < 4.0 / np.sqrt(whole.sum())
+
+
+@pytest.mark.parametrize("key,delta", [(SIG_KEY, None), (ALB_KEY, None)])
+def test_adjoint_matches_oracle_same_samples(cuda, key, delta):
+    # path replay of the NEW state: d<g, img>/d(value) against the float32
+    # oracle with the same samples (both sum per-event terms; the GPU adds
+    # per-lane sums with float atomics)
+    announce("homogeneous adjoint vs oracle", key)
+    sc = rcv.load_dict(homogeneous_box_scene(sigma_t=0.9, albedo=(0.3, 0.6, 0.9), scale=2.0, w=24, h=16))
+    p = rcv.traverse(sc)
+    base = p[key].detach().clone()
+    x = base.clone().requires_grad_(True)
+    for _ in range(2):
+        p[key] = x
+        p.update()
+    gimg = np.random.default_rng(4).uniform(0.5, 1.5, (16, 24, 3)).astype(np.float32)
+    prb = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    g = prb.adjoint(sc, p, torch.from_numpy(gimg).cuda(), 0, 31, 16)[key].cpu().numpy()
+    idx = [pp.key for pp in sc.desc.params].index(key)
+    o = R.adjoint(sc.desc, 8, 8, 31, 16, gimg, param=idx, precision=32)
+    assert g.shape == o.shape
+    np.testing.assert_allclose(g, o, rtol=2e-3, atol=1e-6 * np.abs(o).max())
+
+
+def test_adjoint_closed_form_absorber(cuda):
+    announce("homogeneous adjoint closed form")
+    v, scale, w, h = 0.6, 2.0, 12, 12
+    sc = rcv.load_dict(homogeneous_box_scene(sigma_t=v, albedo=0.0, scale=scale, w=w, h=h, fov=50))
+    p = rcv.traverse(sc)
+    x = p[SIG_KEY].detach().clone().requires_grad_(True)
+    for _ in range(2):
+        p[SIG_KEY] = x
+        p.update()
+    c = pixel_chords(sc.desc.sensors[0].views[0], w, h)
+    gimg = np.random.default_rng(3).uniform(0.5, 1.5, (h, w, 3)).astype(np.float32)
+    expect = float(np.sum(gimg * (-scale * c * np.exp(-scale * v * c)).mean(-1)[..., None]))
+    prb = rcv.load_dict({"type": "prbvolpath", "max_depth": 16, "rr_depth": 16})
+    vals = np.array([float(prb.adjoint(sc, p, torch.from_numpy(gimg).cuda(), 0, 900 + s, 512)[SIG_KEY][0])
+                     for s in range(8)])
+    se = vals.std(ddof=1) / np.sqrt(len(vals))
+    assert abs(vals.mean() - expect) < 4 * se + 1e-3 * abs(expect), (vals, expect)
+
+
+def test_mixed_media_match_oracle(cuda):
+    # a heterogeneous grid cube and a homogeneous sphere in one scene: null
+    # scattering for both media (the homogeneous one keeps its collision as
+    # the surface search bound, :207), analytic NEE in the homogeneous one
+    announce("mixed media vs oracle fp32")
+    g = np.random.default_rng(8).uniform(0.2, 1.0, (6, 6, 6, 1)).astype(np.float32)
+    d = volume_box_scene(g, scale=4.0, w=16, h=12, spp=8, origin=(0.9, 0.6, 2.6))
+    m = {"type": "homogeneous", "sigma_t": 0.7, "albedo": 0.8, "scale": 3.0}
+    d["ball"] = {"type": "sphere", "center": [1.4, 0.5, 0.5], "radius": 0.35, "bsdf": {"type": "null"},
+                 "interior": {"type": "twostatemedium", "old": m, "new": m, "incoming": m}}
+    sc = rcv.load_dict(d)
+    p = _two_state(sc, "ball.interior_medium.sigma_t.value.value", 0.7, 0.9)
+    g2 = (g * 1.2).astype(np.float32)
+    p["box.interior_medium.sigma_t.data"] = torch.from_numpy(g2).cuda()
+    p.update()
+    integ = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 8})
+    H, F = integ.render_twostates(sc, 0, 55, 8)
+    keys = [pp.key for pp in sc.desc.params]
+    po = [pp.init for pp in sc.desc.params]
+    pn = list(po)
+    po[keys.index("ball.interior_medium.sigma_t.value.value")] = np.float32([0.7])
+    pn[keys.index("ball.interior_medium.sigma_t.value.value")] = np.float32([0.9])
+    pn[keys.index("box.interior_medium.sigma_t.data")] = g2
+    Ho, Fo = R.render(sc.desc, VOL, 8, 8, 55, 8, params_old=po, params_new=pn, precision=32)
+    for gg, o in ((H, Ho), (F, Fo)):
+        gg = gg.cpu().numpy()
+        assert _close_fraction(gg, o) > 0.97, _close_fraction(gg, o)
+        assert abs(gg.mean() - o.mean()) < 2e-3 * abs(o.mean()) + 1e-5
+
+
+def test_tutorial_scene_render_matches_oracle(cuda):
+    announce("tutorial scene vs oracle fp32")
+    sc = rcv.load_dict(scenes.homogeneous_cornell_scene_dict(film_res=32, spp=4))
+    _two_state(sc, scenes.HOMOGENEOUS_KEY, 1.5, 2.5)
+    integ = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 16, "rr_depth": 16})
+    H, F = integ.render_twostates(sc, 0, 9, 4)
+    alb = np.float32([0.85, 0.97, 0.999999])
+    Ho, Fo = R.render(sc.desc, VOL, 16, 16, 9, 4, params_old=[np.float32([1.5]), alb],
+                      params_new=[np.float32([2.5]), alb], precision=32)
+    for g, o in ((H, Ho), (F, Fo)):
+        g = g.cpu().numpy()
+        assert _close_fraction(g, o) > 0.97, _close_fraction(g, o)
+
+
+def test_tutorial_optimisation_loop(cuda):
+    # tutorial.ipynb:361-369 with the cv integrator dict of :310-327: Adam lr
+    # 0.1 on sigma_t.value.value from 0.2, clamp [1e-3, 10], L1, spp 2,
+    # spp_grad 16; the target image is a plain render at sigma_t 2.5
+    announce("tutorial optimisation loop")
+    from recursive_control_variates_amd.optim import Adam, l1_loss
+    from recursive_control_variates_amd.rendering import render
+    key = scenes.HOMOGENEOUS_KEY
+    sc = rcv.load_dict(scenes.homogeneous_cornell_scene_dict(film_res=64))
+    plain = rcv.load_dict({"type": "prbvolpath", "max_depth": 16, "rr_depth": 16})
+    ref = plain.render(sc, 0, 12345, 1024)
+    integ = rcv.load_dict({"type": "cv", "integrator": {"type": "twostateprbvolpath", "max_depth": 16,
+                                                          "rr_depth": 16}})
+    params = rcv.traverse(sc)
+    opt = Adam(lr=0.1)
+    opt[key] = torch.tensor([0.2], device="cuda")
+    params.update(opt)
+    losses = []
+    for it in range(60):
+        params.update(opt)
+        img = render(sc, params, integrator=integ, seed=it, spp=2, spp_grad=16)
+        loss = l1_loss(img, ref)
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            opt[key] = torch.clamp(opt[key].detach(), 1e-3, 10)
+        losses.append(float(loss))
+    final = float(opt[key][0])
+    print(f"[tutorial] sigma_t {final:.3f} (target 2.5), loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+    assert np.isfinite(losses).all()
+    assert final > 1.0          # moved from 0.2 towards the target
+    assert np.mean(losses[-10:]) < np.mean(losses[:10])
