@@ -210,6 +210,20 @@ int rcv_render_backward(const rcv_scene *scene,
                         uint32_t seed, int32_t spp, const float *grad_img,
                         int32_t n_params, const int32_t *param_ids,
                         float *const *grad_ptrs, void *stream);
+/* Forward mode: render_forward (plugins/cv_integrator.py:79-85,
+ * plugins/metaintegrator.py:132-138, delegating to the upstream PRB /
+ * PRBVolpath forward mode) of the NEW state. dimg (film_h, film_w, 3) =
+ * d(image)/d(theta) . tangent, where tangents[i] (shaped like parameter
+ * param_ids[i]) is the tangent of that parameter and the others are held
+ * fixed. Paths are replayed with the same random numbers as the primal pass,
+ * so for one seed <g, dimg> equals <adjoint(g), tangent> of
+ * rcv_render_backward up to float rounding. RCV_E_INVALID for
+ * volpathsimple, whose adjoint exists in backward mode only. */
+int rcv_render_forward(const rcv_scene *scene,
+                       const rcv_integrator_desc *integrator, int32_t sensor,
+                       uint32_t seed, int32_t spp, int32_t n_params,
+                       const int32_t *param_ids, const float *const *tangents,
+                       float *dimg, void *stream);
 /* Device event counters of the last render on this scene (tentative
  * collisions, NEE ratio-tracking steps, surface hits, paths): copies
  * `n` int64 counters to host memory `out` (synchronises `stream`). Counting

@@ -637,7 +637,11 @@ struct Scene {
     ds.d = dv / ds.dist;
     R dp = dot(ds.d, ds.n);
     ds.pdf = em_inv_area[idx] * d2 / std::fabs(dp) * sel;
-    if (!(dp < 0) || !(ds.pdf > 0)) return V<R>();
+    // back side or grazing: not a sample (pdf 0, as Mitsuba reports it)
+    if (!(dp < 0) || !(ds.pdf > 0) || !std::isfinite(ds.pdf)) {
+      ds.pdf = 0;
+      return V<R>();
+    }
     return em_rgb(idx) / ds.pdf;
   }
 

@@ -102,6 +102,8 @@ SIGNATURES = {
     "rcv_render_twostates": [_P, C.POINTER(IntegratorDesc), _I, _U32, _I, _P, _P, _P],
     "rcv_render_backward": [_P, C.POINTER(IntegratorDesc), _I, _U32, _I, _P, _I, C.POINTER(_I),
                             C.POINTER(_P), _P],
+    "rcv_render_forward": [_P, C.POINTER(IntegratorDesc), _I, _U32, _I, _I, C.POINTER(_I), C.POINTER(_P), _P,
+                           _P],
     "rcv_cv_step": [C.POINTER(CVState), _P, _P, _I64, _I, _F, _F, _I, _P],
     "rcv_welford_update": [_P, _P, _P, _P, _I64, _F, _F, _P],
     "rcv_loss": [_P, _P, _I64, _I, _P, _P, _P],

@@ -37,7 +37,10 @@ __device__ inline Ray spawn_ray_to(const SurfInt &si, V3 p) {
   return r;
 }
 
-template <int NS, bool REPLAY>
+// REPLAY: the adjoint's replay pass (adj->Lrem starts at the primal
+// radiance) scattering into G; with FWD the replay gathers the tangents in G
+// instead and adds d(L)/d(theta) . tangent to adj->dI.
+template <int NS, bool REPLAY, bool FWD = false>
 __device__ void path_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, Ray ray, V3 L[NS],
                             const LdsStack &stack, LocalCounters &cnt, AdjState *adj, const GradTargets *G) {
   V3 beta[NS];
@@ -125,7 +128,15 @@ __device__ void path_sample(const SceneView &S, const IntegArgs &I, Pcg32 &rng, 
           for (int k = 0; k < NS; ++k) c[k] = beta[k] * mis_em * diffuse_eval(rho[k], wi, wo) * em_w;
         }
       }
-      if (REPLAY && active_next && smooth && bsdf.texture_param >= 0) {
+      if (FWD && active_next && smooth && bsdf.texture_param >= 0) {
+        // the texture tangent at uv, gathered where the adjoint scatters
+        if (const float *tt = G->g[bsdf.texture_param]) {
+          const V3 tv = texture_eval(tt, tf);
+          const V3 r = rho[NS - 1];
+          adj->dI += adj->Lrem * v3(r.x > 0.f ? tv.x / r.x : 0.f, r.y > 0.f ? tv.y / r.y : 0.f,
+                                    r.z > 0.f ? tv.z / r.z : 0.f);
+        }
+      } else if (REPLAY && active_next && smooth && bsdf.texture_param >= 0) {
         // NEE at this vertex and everything after it carry rho_new(uv).
         float *g = G->g[bsdf.texture_param];
         if (g && any_nonzero(adj->Lrem)) {

@@ -153,6 +153,60 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
   if constexpr (INTEG == RCV_INTEGRATOR_VOLPATH && (F & kFeatHomog) != 0) flush_homogeneous(S, G, hom);
 }
 
+// Forward mode (NEW state): per pixel, d(image)/d(theta) . tangent, from the
+// primal pass of each sample and its replay with the same random numbers,
+// which gathers the tangents (in T) where the adjoint scatters (the
+// transpose of adjoint_kernel on the same samples). Launch shape and pixel
+// reduction as render_kernel.
+template <int INTEG, int F>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void forward_kernel(
+    SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int spt, int chunk_log2, GradTargets T,
+    float *__restrict__ dimg) {
+  extern __shared__ int lds_stack[];  // kStackSize ints per thread
+  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  const SensorInfo &sen = S.sensors[sensor];
+  const int film_w = sen.film_w;
+  const int64_t n_pix = int64_t(film_w) * sen.film_h;
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t pixel = tid >> chunk_log2;
+  const int chunk = int(tid & ((1 << chunk_log2) - 1));
+  LocalCounters cnt{S.counters};
+  V3 acc = v3(0.f);
+  if (pixel < n_pix) {
+    const int px = int(pixel % film_w), py = int(pixel / film_w);
+    const int s0 = sample_end(chunk - 1, spp, chunk_log2), s1 = sample_end(chunk, spp, chunk_log2);
+    if constexpr (INTEG == RCV_INTEGRATOR_VOLPATH) {
+      volpath_forward_samples<F>(S, I, sen, seed, px, py, pixel, spp, s0, s1, &T, stack, cnt, acc);
+    } else {
+      for (int s = s0; s < s1; ++s) {
+        Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+        const Ray ray = camera_ray(sen, px, py, rng);
+        const Pcg32 rng0 = rng;
+        V3 L[1];
+        path_sample<1, false>(S, I, rng, ray, L, stack, cnt, nullptr, nullptr);
+        if (!any_nonzero(L[0])) continue;
+        AdjState adj{v3(0.f), L[0], v3(0.f)};
+        rng = rng0;
+        V3 L2[1];
+        path_sample<1, true, true>(S, I, rng, ray, L2, stack, cnt, &adj, &T);
+        acc += adj.dI;
+      }
+    }
+  }
+  for (int off = 1; off < (1 << chunk_log2); off <<= 1) {
+    acc.x += __shfl_xor(acc.x, off, 64);
+    acc.y += __shfl_xor(acc.y, off, 64);
+    acc.z += __shfl_xor(acc.z, off, 64);
+  }
+  if (pixel < n_pix && chunk == 0) {
+    const float inv = float(spp);
+    float *o = dimg + 3 * pixel;
+    o[0] = acc.x / inv;
+    o[1] = acc.y / inv;
+    o[2] = acc.z / inv;
+  }
+}
+
 // Path-replay adjoint of the volume path tracer as a persistent launch: one
 // grid of resident waves whose lanes pull work units (a pixel's chunk of
 // kAdjUnitSamples camera samples) from a device counter, a wavefront's
@@ -625,6 +679,47 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   return check_launch(fn);
 }
 
+int launch_forward(const Scene *s, const rcv_integrator_desc *d, int sensor, uint32_t seed, int spp,
+                   int n_params, const int32_t *param_ids, const float *const *tangents, float *dimg,
+                   hipStream_t stream) {
+  const char *fn = "rcv_render_forward";
+  RCV_REQUIRE(s && s->committed, "%s: scene not committed", fn);
+  RCV_REQUIRE(d, "%s: null integrator", fn);
+  RCV_REQUIRE(sensor >= 0 && sensor < int(s->sensors.size()), "%s: bad sensor %d", fn, sensor);
+  RCV_REQUIRE(spp > 0 && dimg, "%s: bad spp or null output image", fn);
+  RCV_REQUIRE(n_params >= 0 && (n_params == 0 || (param_ids && tangents)), "%s: bad tangent arguments", fn);
+  // volpathsimple's adjoint is written with backward-mode calls only
+  // (dr.backward_from, plugins/volpathsimple.py:183,502,593,620): no forward mode
+  RCV_REQUIRE(d->type == RCV_INTEGRATOR_PATH || d->type == RCV_INTEGRATOR_VOLPATH,
+              "%s: forward mode exists for the path-replay integrators (prb, prbvolpath) only", fn);
+  if (int rc = check_states(s, 1, fn)) return rc;
+  GradTargets T;
+  for (int i = 0; i < kMaxParams; ++i) T.g[i] = nullptr;
+  for (int i = 0; i < n_params; ++i) {
+    RCV_REQUIRE(param_ids[i] >= 0 && param_ids[i] < int(s->params.size()), "%s: bad param id %d", fn, param_ids[i]);
+    RCV_REQUIRE(tangents[i], "%s: null tangent for param %d", fn, param_ids[i]);
+    T.g[param_ids[i]] = const_cast<float *>(tangents[i]);
+  }
+  const SensorInfo &sen = s->sensors[sensor];
+  const int64_t n_pix = int64_t(sen.film_w) * sen.film_h;
+  RCV_REQUIRE(n_pix * spp < (int64_t(1) << 32), "%s: pixels * spp must stay below 2^32", fn);
+  int spt, cl2;
+  if (int rc = plan_launch(d, n_pix, spp, spt, cl2)) return rc;
+  const IntegArgs I{d->max_depth, d->rr_depth, d->hide_emitters, d->flags};
+  const int block = d->block_size ? d->block_size : kDefaultBlock;
+  const unsigned grid = ceil_div(n_pix << cl2, block);
+  if (s->view.counters) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, stream, s->view.counters);
+  if (d->type == RCV_INTEGRATOR_VOLPATH)
+    with_render_features(s->features(), [&](auto f) {
+      hipLaunchKernelGGL((forward_kernel<RCV_INTEGRATOR_VOLPATH, decltype(f)::value>), dim3(grid), dim3(block),
+                         stack_bytes(block), stream, s->view, I, sensor, seed, spp, spt, cl2, T, dimg);
+    });
+  else
+    hipLaunchKernelGGL((forward_kernel<RCV_INTEGRATOR_PATH, 0>), dim3(grid), dim3(block), stack_bytes(block), stream,
+                       s->view, I, sensor, seed, spp, spt, cl2, T, dimg);
+  return check_launch(fn);
+}
+
 namespace {
 __global__ void supergrid_kernel(const float *__restrict__ grid, int nx, int ny, int nz, int sx, int sy, int sz,
                                  float *__restrict__ out) {
@@ -659,6 +754,13 @@ extern "C" int rcv_render_twostates(const rcv_scene *scene, const rcv_integrator
                                     void *stream) {
   return launch_render(static_cast<const Scene *>(scene), integrator, sensor, seed, spp, img_old, img_new, 2,
                        as_stream(stream), "rcv_render_twostates");
+}
+
+extern "C" int rcv_render_forward(const rcv_scene *scene, const rcv_integrator_desc *integrator, int32_t sensor,
+                                  uint32_t seed, int32_t spp, int32_t n_params, const int32_t *param_ids,
+                                  const float *const *tangents, float *dimg, void *stream) {
+  return launch_forward(static_cast<const Scene *>(scene), integrator, sensor, seed, spp, n_params, param_ids,
+                        tangents, dimg, as_stream(stream));
 }
 
 extern "C" int rcv_render_backward(const rcv_scene *scene, const rcv_integrator_desc *integrator, int32_t sensor,

@@ -546,7 +546,13 @@ __device__ inline V3 sample_emitter_direction(const SceneView &S, V3 ref_p, floa
   ds.d = dv / ds.dist;
   const float dp = dot(ds.d, ds.n);
   ds.pdf = e.inv_area * d2 / fabsf(dp) * sel;
-  if (!(dp < 0.0f) || !(ds.pdf > 0.0f)) return v3(0.f);
+  // a point on the emitter's back or in its plane is not a sample: pdf 0,
+  // as Mitsuba reports it (the callers drop lanes with pdf 0,
+  // plugins/twostatevolpath.py:395-397)
+  if (!(dp < 0.0f) || !(ds.pdf > 0.0f) || !(ds.pdf < kInf)) {
+    ds.pdf = 0.0f;
+    return v3(0.f);
+  }
   return emitter_rgb(e) / ds.pdf;
 }
 

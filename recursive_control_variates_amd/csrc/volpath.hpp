@@ -335,12 +335,15 @@ struct NeeRecord {
 // next surface or the end of the segment with the analytic transmittance
 // exp(-sigma_t d) of each state (TwoStateMedium.eval_tr_old/new,
 // plugins/twostatemedium.py:109-115); GRAD adds d(log T)/d(value) * gscale =
-// -scale d gscale to *hom_acc.
+// -scale d gscale to *hom_acc. In forward mode (fwd_sum set) the GRAD pass
+// instead adds d(log T)/d(theta) . tangent (G holds the tangents) to
+// *fwd_sum.
 template <int NS, bool GRAD, int F>
 __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, bool ref_surface, int medium,
                                      Pcg32 &rng, V3 em[NS], DirSample &ds, const LdsStack &stack,
                                      LocalCounters &cnt, float gscale, const GradTargets *G,
-                                     NeeRecord *rec = nullptr, float *hom_acc = nullptr) {
+                                     NeeRecord *rec = nullptr, float *hom_acc = nullptr,
+                                     float *fwd_sum = nullptr) {
   const float u1 = rng.next_1d();
   const float u2 = rng.next_1d();
   const V3 ev = sample_emitter_direction(S, ref_p, u1, u2, ds);
@@ -393,6 +396,8 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
         mev.t = kInf;
         if (rec) rec->hseg += seg;
         if (GRAD && hom_acc && G->g[S.media[medium].sigma_param]) *hom_acc += -gscale * S.media[medium].scale * seg;
+        if (GRAD && fwd_sum)
+          if (const float *tv = G->g[S.media[medium].sigma_param]) *fwd_sum += -S.media[medium].scale * seg * tv[0];
       }
     }
     const bool escaped = active_medium && !mev.valid;
@@ -406,8 +411,12 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
         const MediumInfo &M = S.media[medium];
         float *g = G->g[M.sigma_param];
         if (g && mev.sn[NS - 1] != 0.0f) {
-          scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
-          cnt.add(kCntGradAtomics, 8);
+          if (fwd_sum) {
+            *fwd_sum += -M.scale / mev.sn[NS - 1] * grid_eval1(g, mev.foot);
+          } else {
+            scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
+            cnt.add(kCntGradAtomics, 8);
+          }
         }
       }
       if (rec) {
@@ -449,6 +458,7 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
 struct AdjState {
   V3 dL;    // dLoss/dL of this camera sample
   V3 Lrem;  // radiance still to be gathered along the replayed path
+  V3 dI;    // forward mode: d(radiance)/d(theta) . tangent of this sample
 };
 
 // The collisions of one primal pass of the adjoint: per collision the point
@@ -520,9 +530,14 @@ __device__ inline void flush_homogeneous(const SceneView &S, const GradTargets &
 // (kTapeAdjoint) the primal pass records the collisions and scatters the NEE
 // transmittance gradients at once, and end_adjoint scatters the collisions';
 // otherwise the replay pass (replay = true, the primal radiance in adj.Lrem)
-// replays the same random numbers and scatters as it goes.
-template <int NS, bool ADJ, int F>
+// replays the same random numbers and scatters as it goes. FWD (with ADJ) is
+// forward mode: the replay pass gathers the tangents (in G) where the
+// adjoint scatters and adds d(radiance)/d(theta) . tangent to adj.dI, the
+// transpose of the adjoint on the same samples.
+template <int NS, bool ADJ, int F, bool FWD = false>
 struct VolPath {
+  static_assert(!FWD || ADJ, "forward mode replays paths as the adjoint does");
+  static constexpr bool kTape = ADJ && !FWD && kTapeAdjoint<F>;
   Pcg32 rng;
   Ray ray;
   V3 thr[NS], L[NS];
@@ -532,7 +547,7 @@ struct VolPath {
   int depth, medium;
   bool needs_int, specular_chain, replay, nee_grad;
   AdjState adj;
-  std::conditional_t<ADJ && kTapeAdjoint<F>, PathTape, NoTape> tape;
+  std::conditional_t<kTape, PathTape, NoTape> tape;
   std::conditional_t<ADJ && (F & kFeatHomog) != 0, HomSum, NoHomSum> hom;
 
   // Zeroes the homogeneous-medium sums (once per lane, before its samples).
@@ -580,8 +595,9 @@ struct VolPath {
     if constexpr (ADJ) {
       adj.dL = dL;
       adj.Lrem = Lp;
-      nee_grad = kTapeAdjoint<F> ? !is_replay : is_replay;
-      if constexpr (kTapeAdjoint<F>) tape.n = 0;
+      adj.dI = v3(0.f);
+      nee_grad = kTape ? !is_replay : is_replay;
+      if constexpr (kTape) tape.n = 0;
     }
   }
 
@@ -589,7 +605,7 @@ struct VolPath {
   // done (its taped collisions scattered), false when it needs its replay
   // pass (an untaped kernel, or a tape overflow).
   __device__ bool end_adjoint(const SceneView &S, LocalCounters &cnt, const GradTargets *G) {
-    if constexpr (ADJ && kTapeAdjoint<F>) {
+    if constexpr (kTape) {
       if (replay || !any_nonzero(L[0])) return true;
       if (tape.n <= kTapeRecords) {
         const float total = dot(adj.dL, L[0]);
@@ -732,7 +748,7 @@ struct VolPath {
   #pragma unroll
       for (int k = 0; k < NS; ++k) thr[k] *= w[k];
     }
-    if constexpr (ADJ && kTapeAdjoint<F>) {
+    if constexpr (kTape) {
       if (!replay && (act_null || act_scatter)) {
         // record the collision (its gradient needs the path's radiance)
         const MediumInfo &M = S.media[medium];
@@ -765,11 +781,20 @@ struct VolPath {
         } else {
           coef = -ff_t * M.scale + (act_scatter ? M.scale / mev.st[0] : 0.0f);
         }
-        hom.s += lr * coef;
-        if (act_scatter && M.albedo_param >= 0) {
-          const V3 dl = adj.dL * adj.Lrem;
-          hom.a += v3(mev.alb.x != 0.f ? dl.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? dl.y / mev.alb.y : 0.f,
-                      mev.alb.z != 0.f ? dl.z / mev.alb.z : 0.f);
+        if constexpr (FWD) {
+          if (const float *tv = G->g[M.sigma_param]) adj.dI += adj.Lrem * (coef * tv[0]);
+          if (act_scatter && M.albedo_param >= 0)
+            if (const float *ta = G->g[M.albedo_param])
+              adj.dI += adj.Lrem * v3(mev.alb.x != 0.f ? ta[0] / mev.alb.x : 0.f,
+                                      mev.alb.y != 0.f ? ta[1] / mev.alb.y : 0.f,
+                                      mev.alb.z != 0.f ? ta[2] / mev.alb.z : 0.f);
+        } else {
+          hom.s += lr * coef;
+          if (act_scatter && M.albedo_param >= 0) {
+            const V3 dl = adj.dL * adj.Lrem;
+            hom.a += v3(mev.alb.x != 0.f ? dl.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? dl.y / mev.alb.y : 0.f,
+                        mev.alb.z != 0.f ? dl.z / mev.alb.z : 0.f);
+          }
         }
       }
     }
@@ -779,7 +804,20 @@ struct VolPath {
       const MediumInfo &M = S.media[medium];
       float *gd = G->g[M.sigma_param];
       const float lr = dot(adj.dL, adj.Lrem);
-      if (gd && lr != 0.0f) {
+      if constexpr (FWD) {
+        // the tangent gathered over the same footprint
+        if (gd && !(act_null && mev.sn[0] == 0.0f)) {
+          const float coef = act_null ? -M.scale / mev.sn[0] : M.scale / mev.st[0];
+          adj.dI += adj.Lrem * (coef * grid_eval1(gd, mev.foot));
+        }
+        if ((F & kFeatAlbedoGrid) && act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
+          GridFoot fa;
+          grid_footprint(S.params[M.albedo_param], M, mev.p, fa);
+          const V3 ta = grid_eval3(G->g[M.albedo_param], fa);
+          adj.dI += adj.Lrem * v3(mev.alb.x != 0.f ? ta.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? ta.y / mev.alb.y : 0.f,
+                                  mev.alb.z != 0.f ? ta.z / mev.alb.z : 0.f);
+        }
+      } else if (gd && lr != 0.0f) {
         if (act_null) {
           if (mev.sn[0] != 0.0f) scatter_grid(gd, mev.foot, -lr * M.scale / mev.sn[0]);
         } else {
@@ -787,7 +825,7 @@ struct VolPath {
         }
         cnt.add(kCntGradAtomics, 8);
       }
-      if ((F & kFeatAlbedoGrid) && act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
+      if (!FWD && (F & kFeatAlbedoGrid) && act_scatter && M.albedo_param >= 0 && G->g[M.albedo_param]) {
         const V3 dl = adj.dL * adj.Lrem;
         const V3 ga = v3(mev.alb.x != 0.f ? dl.x / mev.alb.x : 0.f, mev.alb.y != 0.f ? dl.y / mev.alb.y : 0.f,
                          mev.alb.z != 0.f ? dl.z / mev.alb.z : 0.f);
@@ -885,7 +923,38 @@ struct VolPath {
         c[k] = thr[k] * nw[k] * mis_em * em[k];
         L[k] += c[k];
       }
-      if (ADJ && nee_grad) {
+      if constexpr (FWD) {
+        // d(log T)/d(theta) . tangent over the shadow ray, times this NEE
+        // contribution
+        if (nee_grad && any_nonzero(c[0])) {
+          float tsum = 0.0f;
+          if (rec.n <= kNeeRecords) {
+            if constexpr ((F & kFeatHomog) != 0) {
+              if (rec.hseg != 0.0f)
+                if (const float *tv = G->g[S.media[S.hom_medium].sigma_param])
+                  tsum += -S.media[S.hom_medium].scale * rec.hseg * tv[0];
+            }
+            for (int j = 0; j < rec.n; ++j) {
+              const MediumInfo &M = S.media[rec.med[j]];
+              const float *g = G->g[M.sigma_param];
+              const float sn = rec.e[j].w;
+              if (g && sn != 0.0f) {
+                GridFoot f;
+                grid_footprint(S.params[M.sigma_param], M, v3(rec.e[j].x, rec.e[j].y, rec.e[j].z), f);
+                tsum += -M.scale / sn * grid_eval1(g, f);
+              }
+            }
+          } else {
+            Pcg32 r2 = rng_nee;
+            V3 em2[NS];
+            DirSample ds2;
+            nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, 0.0f,
+                                       G, nullptr, nullptr, &tsum);
+          }
+          adj.dI += c[0] * tsum;
+        }
+      }
+      if (ADJ && !FWD && nee_grad) {
         const float cs = dot(adj.dL, c[0]);
         if (cs != 0.0f && any_nonzero(c[0])) {
           if (rec.n <= kNeeRecords) {
@@ -1032,6 +1101,36 @@ __device__ void volpath_adjoint_samples(const SceneView &S, const IntegArgs &I, 
   if constexpr ((F & kFeatHomog) != 0) {
     hom_acc->s += t.hom.s;
     hom_acc->a += t.hom.a;
+  }
+}
+
+// Forward mode of the camera samples [s0, s1) of one pixel (NEW state): per
+// sample the primal pass for its radiance L and, when L is not zero, the
+// replay pass with the same random numbers, which gathers the tangents (in
+// G) where the adjoint scatters; adds d(L)/d(theta) . tangent of each sample
+// to acc.
+template <int F>
+__device__ void volpath_forward_samples(const SceneView &S, const IntegArgs &I, const SensorInfo &sen, uint32_t seed,
+                                        int px, int py, int64_t pixel, int spp, int s0, int s1,
+                                        const GradTargets *G, const LdsStack &stack, LocalCounters &cnt, V3 &acc) {
+  VolPath<1, true, F, true> t;
+  t.L[0] = v3(0.f);  // read (unused) when the first sample starts
+  t.init_hom();
+  bool live = false, want_replay = false;
+  int s = s0;
+  while (true) {
+    if (!live) {
+      if (s >= s1) break;
+      Pcg32 rng = lane_rng(seed, uint32_t(pixel * spp + s));
+      const Ray r = camera_ray(sen, px, py, rng);
+      t.begin_adjoint(rng, r, cnt, want_replay, v3(0.f), t.L[0]);
+    }
+    live = t.bounce(S, I, stack, cnt, G);
+    if (!live) {
+      if (t.replay) acc += t.adj.dI;
+      want_replay = !t.end_adjoint(S, cnt, G);
+      if (!want_replay) ++s;
+    }
   }
 }
 

@@ -70,8 +70,8 @@ class _CVBase:
         return self.state.img.clone()
 
     # adjoint delegation (plugins/cv_integrator.py:79-94)
-    def render_forward(self, scene, params, sensor=0, seed=0, spp=0):
-        return self.adjoint_integrator.render_forward(scene, params, sensor, seed, spp)
+    def render_forward(self, scene, params, sensor=0, seed=0, spp=0, tangents=None):
+        return self.adjoint_integrator.render_forward(scene, params, sensor, seed, spp, tangents)
 
     def render_backward(self, scene, params, grad_in, sensor=0, seed=0, spp=0):
         return self.adjoint_integrator.render_backward(scene, params, grad_in, sensor, seed, spp)

@@ -152,9 +152,35 @@ class RenderIntegrator:
             g = grads[k].to(src.device)
             src.grad = g if src.grad is None else src.grad + g
 
-    def render_forward(self, scene, params, sensor: int = 0, seed: int = 0, spp: int = 0):
-        raise NotImplementedError("forward-mode derivatives are not implemented by this build; "
-                                  "use render_backward")
+    def render_forward(self, scene, params, sensor: int = 0, seed: int = 0, spp: int = 0, tangents=None):
+        """Forward-mode derivative of the NEW-state image: d(image)/d(theta) .
+        tangent. The tangent of each differentiable parameter of ``params``
+        is its ``.grad`` (where Dr.Jit keeps forward-mode gradients as well),
+        or ``tangents[key]`` when given; parameters without one are held
+        fixed. Paths are replayed with the primal's random numbers, so for one
+        seed the result is the transpose of ``adjoint`` on the same samples."""
+        scene.check_params()
+        spp = self._spp(scene, sensor, spp)
+        if tangents is None:
+            tangents = {k: t.grad for k, t in params.differentiable() if t.grad is not None}
+        keys, ptrs, keep = [], [], []
+        for k, t in tangents.items():
+            if k not in scene.params:
+                raise KeyError(f"unknown parameter {k!r}")
+            t = t.detach().to(scene.device, torch.float32).contiguous()
+            if tuple(t.shape) != tuple(scene.params[k].shape):
+                raise ValueError(f"{k}: tangent shape {tuple(t.shape)} != {tuple(scene.params[k].shape)}")
+            keys.append(k)
+            keep.append(t)
+            ptrs.append(t.data_ptr())
+        dimg = torch.empty(scene.film_shape(sensor), dtype=torch.float32, device=scene.device)
+        ids = N.i32_array([scene.params[k].index for k in keys] or [0])
+        arr = (C.c_void_p * max(1, len(keys)))(*ptrs)
+        d = self.desc()
+        N.call("rcv_render_forward", scene.handle, C.byref(d), sensor, C.c_uint32(seed & 0xFFFFFFFF), spp,
+               len(keys), ids, arr, C.c_void_p(dimg.data_ptr()), C.c_void_p(N.stream_handle(scene.device)))
+        del keep
+        return self._post(dimg)
 
 
 class PRBIntegrator(RenderIntegrator):
@@ -189,6 +215,10 @@ class VolpathSimpleIntegrator(RenderIntegrator):
                    | (0 if self.use_drt_subsampling else N.SIMPLE_NO_DRT_SUBSAMPLING)
                    | (0 if self.use_drt_mis else N.SIMPLE_NO_DRT_MIS))
         return d
+
+    def render_forward(self, scene, params, sensor: int = 0, seed: int = 0, spp: int = 0, tangents=None):
+        raise NotImplementedError("volpathsimple has no forward mode: its adjoint is written with backward-mode "
+                                  "calls only (plugins/volpathsimple.py:183,502,593,620)")
 
     def __repr__(self):
         return f"VolpathSimpleIntegrator[max_depth = {self.max_depth}, use_drt = {self.use_drt}]"

@@ -278,6 +278,16 @@ def test_integration_binding_matches_native_structs():
     body = re.search(r"typedef struct rcv_integrator_desc \{(.*?)\} rcv_integrator_desc;", hdr, re.S).group(1)
     names = re.findall(r"int32_t (\w+);", body)
     assert names == [n for n, _ in shown]
+    # the medium descriptor likewise (field names, order and sizes)
+    m = re.search(r"class MediumDesc\(C\.Structure\):.*?\n(.*?)\n\n", text, re.S)
+    assert m, "INTEGRATION.md lost its MediumDesc binding"
+    exec("class MediumDesc(C.Structure):\n" + m.group(1), ns)
+    shown = [(n, t) for n, t in ns["MediumDesc"]._fields_]
+    ref = list(_native.MediumDesc._fields_)
+    assert [n for n, _ in shown] == [n for n, _ in ref]
+    assert [C.sizeof(t) for _, t in shown] == [C.sizeof(t) for _, t in ref]
+    body = re.search(r"typedef struct rcv_medium_desc \{(.*?)\} rcv_medium_desc;", hdr, re.S).group(1)
+    assert re.findall(r"(?:int32_t|float) (\w+)(?:\[3\])?;", body) == [n for n, _ in shown]
 
 
 def test_run_opt_rejects_unsupported_keys():

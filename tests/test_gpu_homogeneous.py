@@ -71,7 +71,9 @@ def test_absorber_closed_form(cuda):
     c = pixel_chords(sc.desc.sensors[0].views[0], w, h)
     expect = np.exp(-scale * v * c).mean(-1)
     whole = (c > 0).all(-1)
-    se = imgs.std(0, ddof=1) / np.sqrt(len(imgs)) + 1e-7
+    # each sample is 0 (absorbed) or 1 (escaped, weight exp(-sigma t) / pdf
+    # = 1): Bernoulli variance of the mean of 8 x spp samples
+    se = np.sqrt(expect * (1 - expect) / (len(imgs) * spp)) + 1e-7
     z = np.where(whole, (imgs.mean(0) - expect) / se, 0.0)
     assert np.abs(z).max() < 5.5, np.abs(z).max()
     assert abs(z[whole].mean()) < 4.0 / np.sqrt(whole.sum())

@@ -254,3 +254,23 @@ def test_colour_adjoints_are_linear_in_the_image_gradient():
     assert np.abs(full).max() > 0
     zero = R.adjoint(d, 4, 100, 7, 64, np.zeros_like(gimg), param=param, precision=64, integrator_type=PATH)
     assert not zero.any()
+
+
+def test_volume_integrator_on_surfaces_matches_path_integrator():
+    # With no medium, the volume integrator is a surface path tracer with NEE
+    # and MIS; with no depth limit both are unbiased for the same light
+    # transport. Emitter samples on the back or in the plane of the light
+    # (NEE from the light itself) have pdf 0, as Mitsuba reports them, so no
+    # value is NaN.
+    from recursive_control_variates_amd import scenes
+    d = _desc(scenes.cornell_scene_dict(np.full((4, 4, 3), 0.5, np.float32), film_res=8))
+    spp = 2048
+    for depth in (2, 3):
+        _, V = R.render(d, VOL, depth, 100, 3, 16, precision=64)
+        assert np.isfinite(V).all(), depth
+    P, _, vP, _ = R.render(d, PATH, -1, 3, 5, spp, precision=64, with_variance=True)
+    _, V, _, vV = R.render(d, VOL, -1, 3, 6, spp, precision=64, with_variance=True)
+    assert np.isfinite(P).all() and np.isfinite(V).all()
+    z = (P - V) / np.sqrt((vP + vV) / spp + 1e-12)
+    assert np.abs(z).max() < 5.5, np.abs(z).max()
+    assert abs(z.mean()) < 0.5
