@@ -281,6 +281,16 @@ int rcv_welford_update(float *mean, float *var, int32_t *counter,
  * grad_out = dloss/dimg (may be NULL). loss_out is a device float. */
 int rcv_loss(const float *img, const float *ref, int64_t n_elem, int32_t kind,
              float *loss_out, float *grad_out, void *stream);
+/* rcv_loss without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory at call time (code under HIP
+ * graph capture, tight serving loops). Same arguments and results (bit for
+ * bit) as rcv_loss; its partial sums live in a library-owned static device
+ * array of 256 floats + a counter per device instead of a workspace that
+ * rcv_loss allocates on its first call. Calls that overlap on one device
+ * (concurrent streams) race on that array and must be ordered. */
+int rcv_loss_noalloc(const float *img, const float *ref, int64_t n_elem,
+                     int32_t kind, float *loss_out, float *grad_out,
+                     void *stream);
 /* Adam (mi.ad.Adam as used at optimize.py:90,127) fused with clamp
  * (utils.py:172-175): param_out = clamp(param_in - lr_t*m/(sqrt(v)+eps)),
  * with m, v updated in place; t is the 1-based step count. param_out may

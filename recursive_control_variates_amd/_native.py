@@ -107,6 +107,7 @@ SIGNATURES = {
     "rcv_cv_step": [C.POINTER(CVState), _P, _P, _I64, _I, _F, _F, _I, _P],
     "rcv_welford_update": [_P, _P, _P, _P, _I64, _F, _F, _P],
     "rcv_loss": [_P, _P, _I64, _I, _P, _P, _P],
+    "rcv_loss_noalloc": [_P, _P, _I64, _I, _P, _P, _P],
     "rcv_adam_step": [_P, _P, _P, _P, _P, _I64, _I, _F, _F, _F, _F, _F, _F, _P],
 }
 

@@ -113,10 +113,66 @@ __global__ __launch_bounds__(256) void adam_kernel(float *p_out, const float *p_
   p_out[i] = fminf(fmaxf(upd, lo), hi);
 }
 
+// Scratch of rcv_loss_noalloc: the per-block partial sums and the
+// last-block counter of loss_kernel in a file-scope device array instead of
+// a library allocation, so the entry point allocates nothing. Every device
+// the library runs on holds its own copy (a few KiB) until the process ends.
+// Host code never passes the array's name to a kernel: it takes the device
+// address from hipGetSymbolAddress, once per device.
+constexpr int kNoAllocScratchFloats = kLossBlocks + 64;  // partials, then the counter (padded)
+
+__global__ void zero_scratch_kernel(float *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0f;
+}
+
+}  // namespace
+
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace {
+// Device address of `scratch` on the current device, looked up at the first
+// call on that device (with it current) and kept per device.
+int noalloc_scratch(float *&out) {
+  static std::mutex mu;
+  static float *ptr[64] = {};
+  int dev = 0;
+  RCV_TRY_HIP(hipGetDevice(&dev), "rcv_loss_noalloc: hipGetDevice");
+  RCV_REQUIRE(dev >= 0 && dev < 64, "rcv_loss_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    RCV_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "rcv_loss_noalloc: hipGetSymbolAddress");
+    RCV_REQUIRE(p, "rcv_loss_noalloc: no device address for the scratch array");
+    ptr[dev] = static_cast<float *>(p);
+  }
+  out = ptr[dev];
+  return RCV_OK;
+}
+
 }  // namespace
 }  // namespace rcv
 
 using namespace rcv;
+
+extern "C" int rcv_loss_noalloc(const float *img, const float *ref, int64_t n_elem, int32_t kind, float *loss_out,
+                                float *grad_out, void *stream) {
+  RCV_REQUIRE(img && ref && loss_out, "rcv_loss_noalloc: null pointer");
+  RCV_REQUIRE(n_elem > 0, "rcv_loss_noalloc: n_elem must be positive");
+  RCV_REQUIRE(kind == RCV_LOSS_L1 || kind == RCV_LOSS_L2, "rcv_loss_noalloc: unknown loss kind %d", kind);
+  const unsigned grid = unsigned(std::min<int64_t>(kLossBlocks, (n_elem + kLossThreads - 1) / kLossThreads));
+  // the scratch holds kLossBlocks partial sums; the grid never exceeds it
+  RCV_REQUIRE(int(grid) <= kLossBlocks, "rcv_loss_noalloc: %u blocks exceed the scratch of %d partial sums", grid,
+              kLossBlocks);
+  float *ws = nullptr;
+  if (int rc = noalloc_scratch(ws)) return rc;
+  // the last-block counter must start at zero (the array keeps the previous
+  // call's contents); a fill kernel, not a memset, so the call can be captured
+  // in a graph. The partial sums are written before they are read.
+  hipLaunchKernelGGL(zero_scratch_kernel, dim3(1), dim3(64), 0, as_stream(stream), ws + kLossBlocks, 64);
+  hipLaunchKernelGGL(loss_kernel, dim3(grid), dim3(kLossThreads), 0, as_stream(stream), img, ref, n_elem, kind,
+                     loss_out, grad_out, ws, reinterpret_cast<unsigned *>(ws + kLossBlocks));
+  return check_launch("rcv_loss_noalloc");
+}
 
 extern "C" int rcv_loss(const float *img, const float *ref, int64_t n_elem, int32_t kind, float *loss_out,
                         float *grad_out, void *stream) {
