@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--max-depth", type=int, default=16)
+    ap.add_argument("--majorant-factor", default="0",
+                    help="majorant supergrid of the volume workload: 0 = one global majorant, k > 0 = cells of k^3 "
+                         "voxels, 'auto' = the reference's rule for use_majorant_supergrid (utils.py:27-43)")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--adjoint", choices=("prb", "volpathsimple"), default="prb",
                     help="adjoint integrator of the step: path replay of the primal integrator, or "
@@ -110,6 +113,12 @@ def build_problem(args, method="cv_ps"):
     init = np.full_like(target, 0.1)
     sc = rcv.load_dict(scenes.volume_scene_dict(init, twostate=(method == "cv_ps"), film_res=args.res,
                                                 spp=args.spp))
+    mf = getattr(args, "majorant_factor", "0")
+    if mf == "auto":
+        from recursive_control_variates_amd.optimize import adjust_majorant_res_factor
+        adjust_majorant_res_factor(sc, init.shape)
+    elif int(mf) > 0:
+        sc.set_majorant_factor(0, int(mf))
     itype = "twostateprbvolpath" if method == "cv_ps" else "prbvolpath"
     integ = rcv.load_dict({"type": itype, "max_depth": args.max_depth, "rr_depth": args.max_depth})
     return sc, integ, torch.from_numpy(target).cuda(), torch.from_numpy(init).cuda()
@@ -317,6 +326,10 @@ def rooflines(run, counts, window):
                      "frac": round(float(ach) / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_state": tstate,
                      "kernel": kern, "sampled_steps": [i + 1 for i in idx],
                      "kernel_ms_sampled_mean": round(float(ms.mean()), 4),
+                     "kernel_ms_sampled": [round(float(x), 3) for x in ms],
+                     "events_sampled": [{k: counts[i][bkey.replace("bytes", "counters")][k]
+                                         for k in ("collisions", "nee_steps", "grad_atomics", "null_events",
+                                                   "real_scatter")} for i in idx],
                      "kernel_ms_window_mean": round(float(run[mkey][first:first + count].mean()), 4),
                      "alg_bytes_per_launch_mean": float(by.mean())}
     return out
@@ -353,6 +366,17 @@ def supergrid_compare(args, st, factor=8):
                               "twostate_collisions": rc["collisions"], "twostate_null_events": rc["null_events"],
                               "adjoint_collisions": ac["collisions"], "adjoint_nee_steps": ac["nee_steps"]}
     sc.set_majorant_factor(0, 0)
+    # where the adjoint's time goes at this state: the same launch without
+    # gradient targets (primal and replay passes, no scatter-adds) and the
+    # plain render of the same samples
+    x = st.opt[st.key].detach().clone()
+    for _ in range(2):
+        st.params[st.key] = x
+        st.params.update()
+    out["adjoint_without_scatter_ms"] = round(_event_time(
+        lambda r: adj.adjoint(sc, st.params, gimg, 0, 4000 + r, args.spp), args.kernel_reps), 3)
+    out["plain_render_same_samples_ms"] = round(_event_time(
+        lambda r: adj.render(sc, 0, 4000 + r, args.spp), args.kernel_reps), 3)
     out["state"] = "the parameters after the last step of the cv_ps run"
     return out
 
@@ -498,7 +522,7 @@ def bench_surface(args):
     init = np.full_like(target, 0.5)
     obs = torch.from_numpy(observable_texels(64)).cuda()
     tgt = torch.from_numpy(target).cuda()
-    levels = (1e-2, 5e-3, 2e-3, 1e-3)
+    levels = (3e-2, 2e-2, 1.5e-2, 1e-2, 5e-3, 1e-3)
     out = {"workload": "Cornell box 128^2, 64^2 x 3 texture, max_depth 4, spp 16/16, Adam lr 0.02, L2, "
                        "50 iterations", "observable_texels": int(obs.sum()), "texels": 64 * 64}
     img_ref = None
@@ -541,9 +565,10 @@ def bench_surface(args):
                        "by_level": time_to_levels(run, levels),
                        "by_level_observable": time_to_levels(run, levels, mo),
                        "final_mse": float(run["mse"][-1]), "final_mse_observable": float(mo[-1]),
-                       "ms_per_step_mean": round(float(run["per"].mean()), 4),
-                       "p50_step_ms": round(float(np.percentile(run["per"], 50)), 4),
-                       "p99_step_ms": round(float(np.percentile(run["per"], 99)), 4)}
+                       "ms_per_step_after_skip": round(float(run["per"][SKIP_STEPS:].mean()), 4),
+                       "p50_step_ms_after_skip": round(float(np.percentile(run["per"][SKIP_STEPS:], 50)), 4),
+                       "p99_step_ms_after_skip": round(float(np.percentile(run["per"][SKIP_STEPS:], 99)), 4),
+                       "run_seconds": round(float(run["cum"][-1]) / 1e3, 4)}
         log(f"[bench] surface {method}: {out[method]}")
     return out
 
