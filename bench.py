@@ -255,7 +255,7 @@ def count_bytes(args, st, run):
     two-state: 2 states x 8 trilinear corners x 4 B per tentative collision
     and per ratio-tracking step, 2 images x 12 B per pixel;
     adjoint: 1 state x 8 corners x 4 B per lookup in the primal and replay
-    passes and per recorded NEE collision, 4 B per gradient scatter-add,
+    passes (main path and NEE ratio tracking), 4 B per gradient scatter-add,
     12 B per pixel of the image gradient read."""
     sc, p, key = st.sc, st.params, st.key
     n_pix = sc.film_shape(0)[0] * sc.film_shape(0)[1]

@@ -313,17 +313,66 @@ __device__ inline bool null_bsdf(const SceneView &S, int shape) {
   return b >= 0 && S.bsdfs[b].type == RCV_BSDF_NULL;
 }
 
-// The shadow-ray collisions of one NEE (point, sigma_n of the NEW state,
-// medium) recorded by the replay pass, so that the transmittance gradient can
-// be scattered once the NEE contribution is known without tracking the
-// shadow ray a second time; n counts every collision, and a shadow ray with
-// more than kNeeRecords of them is tracked again instead.
+// A recorded collision keeps the trilinear footprint of its density lookup
+// instead of its point, so that reading it back needs no world-to-grid
+// transform and no load of its medium's description: the three fractions
+// and the collision's gradient coefficient in a float4, and one word packing
+// the base voxel (kRecBaseBits bits), the +1 flag of each axis (clear where
+// the footprint is clamped) and the medium. A footprint that does not pack
+// (a grid of more than 2^26 voxels, a medium index above 7) overflows the
+// record, and the shadow ray or path is tracked again instead.
+constexpr int kRecBaseBits = 26;
+constexpr unsigned kRecBaseMask = (1u << kRecBaseBits) - 1u;
+
+__device__ inline bool pack_cell(const GridFoot &f, int medium, unsigned &cell) {
+  if (unsigned(f.base) > kRecBaseMask || unsigned(medium) > 7u) return false;
+  cell = unsigned(f.base) | (f.dx != 0 ? 1u << 26 : 0u) | (f.dy != 0 ? 1u << 27 : 0u) |
+         (f.dz != 0 ? 1u << 28 : 0u) | (unsigned(medium) << 29);
+  return true;
+}
+
+__device__ inline int cell_medium(unsigned cell) { return int(cell >> 29); }
+
+// The medium of the records being read back: its gradient (or tangent)
+// buffer and the row and slice strides of its density grid, loaded again
+// only when a record names another medium.
+struct RecMedium {
+  int med = -1;
+  float *g = nullptr;
+  int sy = 0, sz = 0;
+  __device__ void select(const SceneView &S, const GradTargets *G, int m) {
+    if (m == med) return;
+    med = m;
+    const int p = S.media[m].sigma_param;
+    g = G->g[p];
+    sy = S.params[p].nx;
+    sz = S.params[p].nx * S.params[p].ny;
+  }
+  __device__ GridFoot foot(float4 e, unsigned cell) const {
+    GridFoot f;
+    f.base = int(cell & kRecBaseMask);
+    f.dx = int((cell >> 26) & 1u);
+    f.dy = ((cell >> 27) & 1u) ? sy : 0;
+    f.dz = ((cell >> 28) & 1u) ? sz : 0;
+    f.tx = e.x;
+    f.ty = e.y;
+    f.tz = e.z;
+    return f;
+  }
+};
+
+// The shadow-ray collisions of one NEE (footprint, -scale / sigma_n of the
+// NEW state, 0 where sigma_n = 0) recorded by the replay pass, so that the
+// transmittance gradient can be scattered once the NEE contribution is known
+// without tracking the shadow ray a second time; n counts the records and is
+// kNeeRecords + 1 once they overflow (more collisions than records, or a
+// footprint that does not pack), and such a shadow ray is tracked again.
 constexpr int kNeeRecords = 32;
 struct NeeRecord {
   int n;
   float hseg;  // length of the shadow ray inside the homogeneous medium
   float4 e[kNeeRecords];
-  int med[kNeeRecords];
+  unsigned cell[kNeeRecords];
 };
 
 // sample_emitter_twostates (plugins/twostatevolpath.py:387-479): emitter
@@ -407,7 +456,7 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
       si.t = si.t - mev.t;
 #pragma unroll
       for (int k = 0; k < NS; ++k) trm[k] = mev.maj != 0.0f ? mev.sn[k] / mev.maj : mev.sn[k];
-      if (GRAD) {
+      if (GRAD && (fwd_sum || gscale != 0.0f)) {
         const MediumInfo &M = S.media[medium];
         float *g = G->g[M.sigma_param];
         if (g && mev.sn[NS - 1] != 0.0f) {
@@ -420,11 +469,16 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
         }
       }
       if (rec) {
-        if (rec->n < kNeeRecords) {
-          rec->e[rec->n] = make_float4(mev.p.x, mev.p.y, mev.p.z, mev.sn[NS - 1]);
-          rec->med[rec->n] = medium;
+        unsigned cell;
+        if (rec->n < kNeeRecords && pack_cell(mev.foot, medium, cell)) {
+          const float sn = mev.sn[NS - 1];
+          rec->e[rec->n] = make_float4(mev.foot.tx, mev.foot.ty, mev.foot.tz,
+                                       sn != 0.0f ? -S.media[medium].scale / sn : 0.0f);
+          rec->cell[rec->n] = cell;
+          rec->n++;
+        } else {
+          rec->n = kNeeRecords + 1;
         }
-        rec->n++;
       }
     }
     active_surface = (active_surface || escaped) && si.valid && !active_medium;
@@ -472,11 +526,24 @@ struct AdjState {
 constexpr int kTapeRecords = 32;
 struct PathTape {
   int n;
-  float4 e[kTapeRecords];  // point, coefficient
-  float s[kTapeRecords];   // dLoss/dL . L gathered before the collision
-  int med[kTapeRecords];
+  float4 e[kTapeRecords];      // footprint fractions, coefficient
+  float s[kTapeRecords];       // dLoss/dL . L gathered before the collision
+  unsigned cell[kTapeRecords];  // packed footprint and medium (pack_cell)
 };
 struct NoTape {};
+
+// Untaped adjoint: dLoss/dL . c of each NEE contribution c of the primal
+// pass, in path order. The replay pass reads the value before it tracks the
+// same shadow ray (same random numbers, same collisions), so it scatters the
+// transmittance gradient while tracking: one shadow-ray tracking per pass and
+// nothing recorded per collision. A path with more NEEs than kNeeSlots
+// tracks the later ones' shadow rays again in the replay.
+constexpr int kNeeSlots = 32;
+struct NeeWeights {
+  int k;
+  float cs[kNeeSlots];
+};
+struct NoNeeWeights {};
 
 // One-pass (taped) adjoint: for kernels whose media have a majorant
 // supergrid and whose only medium gradient is the density's. Records live in
@@ -548,6 +615,8 @@ struct VolPath {
   bool needs_int, specular_chain, replay, nee_grad;
   AdjState adj;
   std::conditional_t<kTape, PathTape, NoTape> tape;
+  static constexpr bool kNeeInline = ADJ && !FWD && !kTape;
+  std::conditional_t<kNeeInline, NeeWeights, NoNeeWeights> neew;
   std::conditional_t<ADJ && (F & kFeatHomog) != 0, HomSum, NoHomSum> hom;
 
   // Zeroes the homogeneous-medium sums (once per lane, before its samples).
@@ -598,6 +667,7 @@ struct VolPath {
       adj.dI = v3(0.f);
       nee_grad = kTape ? !is_replay : is_replay;
       if constexpr (kTape) tape.n = 0;
+      if constexpr (kNeeInline) neew.k = 0;
     }
   }
 
@@ -609,14 +679,14 @@ struct VolPath {
       if (replay || !any_nonzero(L[0])) return true;
       if (tape.n <= kTapeRecords) {
         const float total = dot(adj.dL, L[0]);
+        RecMedium rm;
         for (int j = 0; j < tape.n; ++j) {
           const float lr = total - tape.s[j];
           if (lr != 0.0f) {
-            const MediumInfo &M = S.media[tape.med[j]];
             const float4 e = tape.e[j];
-            GridFoot f;
-            grid_footprint(S.params[M.sigma_param], M, v3(e.x, e.y, e.z), f);
-            scatter_grid(G->g[M.sigma_param], f, lr * e.w);
+            const unsigned cell = tape.cell[j];
+            rm.select(S, G, cell_medium(cell));
+            scatter_grid(rm.g, rm.foot(e, cell), lr * e.w);
             cnt.add(kCntGradAtomics, 8);
           }
         }
@@ -753,13 +823,16 @@ struct VolPath {
         // record the collision (its gradient needs the path's radiance)
         const MediumInfo &M = S.media[medium];
         if (G->g[M.sigma_param] && !(act_null && mev.sn[0] == 0.0f)) {
-          if (tape.n < kTapeRecords) {
+          unsigned cell;
+          if (tape.n < kTapeRecords && pack_cell(mev.foot, medium, cell)) {
             const float coef = act_null ? -M.scale / mev.sn[0] : M.scale / mev.st[0];
-            tape.e[tape.n] = make_float4(mev.p.x, mev.p.y, mev.p.z, coef);
+            tape.e[tape.n] = make_float4(mev.foot.tx, mev.foot.ty, mev.foot.tz, coef);
             tape.s[tape.n] = dot(adj.dL, L[0]);
-            tape.med[tape.n] = medium;
+            tape.cell[tape.n] = cell;
+            tape.n++;
+          } else {
+            tape.n = kTapeRecords + 1;
           }
-          tape.n++;
         }
       }
     }
@@ -898,11 +971,33 @@ struct VolPath {
       V3 em[NS];
       DirSample ds;
       const Pcg32 rng_nee = rng;
-      NeeRecord rec;
-      rec.n = 0;
-      rec.hseg = 0.0f;
-      nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G,
-                                  ADJ && nee_grad ? &rec : nullptr);
+      // Untaped adjoint, replay pass: this NEE's dLoss/dL . c from the primal
+      // pass, with which the tracking below scatters the transmittance
+      // gradient.
+      float cs_in = 0.0f;
+      bool have_cs = false;
+      if constexpr (kNeeInline) {
+        if (replay && neew.k < kNeeSlots) {
+          cs_in = neew.cs[neew.k];
+          have_cs = true;
+        }
+      }
+      float tsum = 0.0f;  // forward mode: d(log T)/d(theta) . tangent along the shadow ray
+      NeeRecord rec;      // taped adjoint: the shadow ray's collisions
+      if constexpr (kTape) {
+        rec.n = 0;
+        rec.hseg = 0.0f;
+      }
+      if constexpr (ADJ) {
+        // one instantiation for both passes, so that they compute the same
+        // transmittance from the same random numbers
+        float *hacc = nullptr;
+        if constexpr (kNeeInline && (F & kFeatHomog) != 0) hacc = have_cs ? &hom.s : nullptr;
+        nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, cs_in, G,
+                                   kTape && nee_grad ? &rec : nullptr, hacc, FWD && nee_grad ? &tsum : nullptr);
+      } else {
+        nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G);
+      }
       V3 nw[NS];
       float nee_pdf;
       if (active_e_surface) {
@@ -924,52 +1019,43 @@ struct VolPath {
         L[k] += c[k];
       }
       if constexpr (FWD) {
-        // d(log T)/d(theta) . tangent over the shadow ray, times this NEE
+        // the shadow ray's d(log T)/d(theta) . tangent times this NEE
         // contribution
-        if (nee_grad && any_nonzero(c[0])) {
-          float tsum = 0.0f;
-          if (rec.n <= kNeeRecords) {
-            if constexpr ((F & kFeatHomog) != 0) {
-              if (rec.hseg != 0.0f)
-                if (const float *tv = G->g[S.media[S.hom_medium].sigma_param])
-                  tsum += -S.media[S.hom_medium].scale * rec.hseg * tv[0];
-            }
-            for (int j = 0; j < rec.n; ++j) {
-              const MediumInfo &M = S.media[rec.med[j]];
-              const float *g = G->g[M.sigma_param];
-              const float sn = rec.e[j].w;
-              if (g && sn != 0.0f) {
-                GridFoot f;
-                grid_footprint(S.params[M.sigma_param], M, v3(rec.e[j].x, rec.e[j].y, rec.e[j].z), f);
-                tsum += -M.scale / sn * grid_eval1(g, f);
-              }
-            }
-          } else {
+        if (nee_grad && any_nonzero(c[0])) adj.dI += c[0] * tsum;
+      }
+      if constexpr (kNeeInline) {
+        if (!replay) {
+          if (neew.k < kNeeSlots) neew.cs[neew.k] = dot(adj.dL, c[0]);
+        } else if (!have_cs) {
+          // past the slots: track the shadow ray again, now with its weight
+          const float cs = dot(adj.dL, c[0]);
+          if (cs != 0.0f && any_nonzero(c[0])) {
             Pcg32 r2 = rng_nee;
             V3 em2[NS];
             DirSample ds2;
-            nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, 0.0f,
-                                       G, nullptr, nullptr, &tsum);
+            float *hacc = nullptr;
+            if constexpr ((F & kFeatHomog) != 0) hacc = &hom.s;
+            nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G,
+                                       nullptr, hacc);
           }
-          adj.dI += c[0] * tsum;
         }
+        ++neew.k;
       }
-      if (ADJ && !FWD && nee_grad) {
+      if constexpr (kTape) {
         const float cs = dot(adj.dL, c[0]);
-        if (cs != 0.0f && any_nonzero(c[0])) {
+        if (nee_grad && cs != 0.0f && any_nonzero(c[0])) {
           if (rec.n <= kNeeRecords) {
-            if constexpr (ADJ && (F & kFeatHomog) != 0) {
+            if constexpr ((F & kFeatHomog) != 0) {
               if (rec.hseg != 0.0f) hom.s += -cs * S.media[S.hom_medium].scale * rec.hseg;
             }
             // d(log T)/d(density) at each recorded shadow-ray collision
+            RecMedium rm;
             for (int j = 0; j < rec.n; ++j) {
-              const MediumInfo &M = S.media[rec.med[j]];
-              float *g = G->g[M.sigma_param];
-              const float sn = rec.e[j].w;
-              if (g && sn != 0.0f) {
-                GridFoot f;
-                grid_footprint(S.params[M.sigma_param], M, v3(rec.e[j].x, rec.e[j].y, rec.e[j].z), f);
-                scatter_grid(g, f, -cs * M.scale / sn);
+              const float4 e = rec.e[j];
+              const unsigned cell = rec.cell[j];
+              rm.select(S, G, cell_medium(cell));
+              if (rm.g && e.w != 0.0f) {
+                scatter_grid(rm.g, rm.foot(e, cell), cs * e.w);
                 cnt.add(kCntGradAtomics, 8);
               }
             }
@@ -977,12 +1063,10 @@ struct VolPath {
             Pcg32 r2 = rng_nee;
             V3 em2[NS];
             DirSample ds2;
-            if constexpr (ADJ && (F & kFeatHomog) != 0)
-              nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs,
-                                         G, nullptr, &hom.s);
-            else if constexpr (ADJ)
-              nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs,
-                                         G);
+            float *hacc = nullptr;
+            if constexpr ((F & kFeatHomog) != 0) hacc = &hom.s;
+            nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G,
+                                       nullptr, hacc);
           }
         }
       }

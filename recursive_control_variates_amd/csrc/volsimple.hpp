@@ -94,11 +94,15 @@ __device__ inline float simple_ratio_tracking(const SceneView &S, int medium, Ra
       }
     }
     if (rec) {
-      if (rec->n < kNeeRecords) {
-        rec->e[rec->n] = make_float4(ev.p.x, ev.p.y, ev.p.z, tr > 0.0f ? ev.sn[0] : 0.0f);
-        rec->med[rec->n] = medium;
+      unsigned cell;
+      if (rec->n < kNeeRecords && pack_cell(ev.foot, medium, cell)) {
+        rec->e[rec->n] = make_float4(ev.foot.tx, ev.foot.ty, ev.foot.tz,
+                                     tr > 0.0f ? -S.media[medium].scale / ev.sn[0] : 0.0f);
+        rec->cell[rec->n] = cell;
+        rec->n++;
+      } else {
+        rec->n = kNeeRecords + 1;
       }
-      rec->n++;
     }
     T *= tr;
     ray.o = ev.p;
@@ -332,13 +336,12 @@ __device__ V3 simple_sample(const SceneView &S, const IntegArgs &I, int medium, 
           DirSample ds2;
           if (rec.n <= kNeeRecords) {
             // the replay's scatter-adds, from the recorded collisions
-            float *g = G->g[M.sigma_param];
-            for (int j = 0; g && j < rec.n; ++j) {
-              const float sn = rec.e[j].w;
-              if (sn != 0.0f) {
-                GridFoot f;
-                grid_footprint(S.params[M.sigma_param], M, v3(rec.e[j].x, rec.e[j].y, rec.e[j].z), f);
-                scatter_grid(g, f, -cs * M.scale / sn);
+            RecMedium rm;
+            rm.select(S, G, medium);
+            for (int j = 0; rm.g && j < rec.n; ++j) {
+              const float4 e = rec.e[j];
+              if (e.w != 0.0f) {
+                scatter_grid(rm.g, rm.foot(e, rec.cell[j]), cs * e.w);
                 cnt.add(kCntGradAtomics, 8);
               }
             }
