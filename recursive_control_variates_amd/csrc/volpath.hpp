@@ -375,6 +375,144 @@ struct NeeRecord {
   unsigned cell[kNeeRecords];
 };
 
+// A shadow ray of sample_emitter_twostates in flight: its current segment,
+// the distance tracked so far, its medium and the two transmittances.
+template <int NS>
+struct Shadow {
+  Ray ray;
+  float total;
+  int medium;
+  bool needs_int;
+  float T[NS];
+};
+
+// Emitter direction without visibility (plugins/twostatevolpath.py:397-405)
+// and the shadow ray towards it; returns the emitter's value / pdf (0 and
+// ds.pdf = 0 when the sample is invalid: nothing to track).
+template <int NS>
+__device__ inline V3 shadow_begin(const SceneView &S, V3 ref_p, V3 ref_n, bool ref_surface, int medium, Pcg32 &rng,
+                                  DirSample &ds, Shadow<NS> &sh) {
+  const float u1 = rng.next_1d();
+  const float u2 = rng.next_1d();
+  const V3 ev = sample_emitter_direction(S, ref_p, u1, u2, ds);
+  sh.ray.o = ref_surface ? offset_point(ref_p, ref_n, ds.d) : ref_p;
+  sh.ray.d = ds.d;
+  sh.ray.maxt = kInf;
+  sh.total = 0.0f;
+  sh.medium = medium;
+  sh.needs_int = true;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) sh.T[k] = 1.0f;
+  return ev;
+}
+
+// One step of the ratio tracking of a shadow ray towards an emitter sample
+// at distance `dist` (the body of the loop of :409-477): a collision in the
+// medium (transmittance ratio sigma_n / majorant per state) or the next
+// surface (null: crossed; otherwise the transmittance is 0). False once the
+// shadow ray has reached the sample or both transmittances are 0. `si` holds
+// the shadow ray's next surface between steps. GRAD: see nee_twostates.
+template <int NS, bool GRAD, int F>
+__device__ inline bool shadow_step(const SceneView &S, Shadow<NS> &sh, SurfInt &si, float dist, Pcg32 &rng,
+                                   const LdsStack &stack, LocalCounters &cnt, float gscale, const GradTargets *G,
+                                   NeeRecord *rec, float *hom_acc, float *fwd_sum) {
+  const float remaining = dist * (1.0f - kShadowEps) - sh.total;
+  sh.ray.maxt = remaining;
+  if (!(remaining > 0.0f)) return false;
+  if (sh.needs_int) {
+    Hit h;
+    trace<false>(S, sh.ray, h, stack);
+    cnt.add(kCntTraversals, 1);
+    si = make_si(S, sh.ray, h);
+    sh.needs_int = false;
+  }
+  bool active_medium = sh.medium >= 0;
+  bool active_surface = !active_medium;
+  MediumEv<NS> mev;
+  mev.valid = false;
+  mev.t = kInf;
+  float trm[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) trm[k] = 1.0f;
+  if (active_medium) {
+    const float u = rng.next_1d();
+    sample_medium<NS, F>(S, sh.medium, sh.ray, u, mev);
+    cnt.add(kCntNeeSteps, 1);
+    if (si.t < mev.t) {
+      mev.valid = false;
+      mev.t = kInf;
+    }
+    if ((F & kFeatHomog) && S.media[sh.medium].homogeneous) {
+      const float seg = fminf(fminf(remaining, si.t), si.t) - mev.mint;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) trm[k] = expf(-(seg * mev.st[k]));
+      mev.valid = false;
+      mev.t = kInf;
+      if (rec) rec->hseg += seg;
+      const MediumInfo &M = S.media[sh.medium];
+      if (GRAD && hom_acc && G->g[M.sigma_param]) *hom_acc += -gscale * M.scale * seg;
+      if (GRAD && fwd_sum)
+        if (const float *tv = G->g[M.sigma_param]) *fwd_sum += -M.scale * seg * tv[0];
+    }
+  }
+  const bool escaped = active_medium && !mev.valid;
+  active_medium = active_medium && mev.valid;
+  if (active_medium) {
+    sh.ray.o = mev.p;
+    si.t = si.t - mev.t;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) trm[k] = mev.maj != 0.0f ? mev.sn[k] / mev.maj : mev.sn[k];
+    if (GRAD && (fwd_sum || gscale != 0.0f)) {
+      const MediumInfo &M = S.media[sh.medium];
+      float *g = G->g[M.sigma_param];
+      if (g && mev.sn[NS - 1] != 0.0f) {
+        if (fwd_sum) {
+          *fwd_sum += -M.scale / mev.sn[NS - 1] * grid_eval1(g, mev.foot);
+        } else {
+          scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
+          cnt.add(kCntGradAtomics, 8);
+        }
+      }
+    }
+    if (rec) {
+      unsigned cell;
+      if (rec->n < kNeeRecords && pack_cell(mev.foot, sh.medium, cell)) {
+        const float sn = mev.sn[NS - 1];
+        rec->e[rec->n] = make_float4(mev.foot.tx, mev.foot.ty, mev.foot.tz,
+                                     sn != 0.0f ? -S.media[sh.medium].scale / sn : 0.0f);
+        rec->cell[rec->n] = cell;
+        rec->n++;
+      } else {
+        rec->n = kNeeRecords + 1;
+      }
+    }
+  }
+  active_surface = (active_surface || escaped) && si.valid && !active_medium;
+  if (active_surface) {
+    const float b = (!(F & kFeatSurfaces) || null_bsdf(S, si.shape)) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) trm[k] = trm[k] * b;
+  }
+  bool any_t = false;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    sh.T[k] *= trm[k];
+    any_t = any_t || sh.T[k] != 0.0f;
+  }
+  if (active_surface) {
+    sh.ray = spawn_ray(si, sh.ray.d);
+    sh.needs_int = true;
+  }
+  const bool active = (active_medium || active_surface) && any_t;
+  if (!active) return false;
+  sh.total += active_medium ? mev.t : si.t;
+  if (active_surface) {
+    const ShapeInfo bs = S.shapes[si.shape];
+    if (bs.interior != bs.exterior) sh.medium = dot(si.n, sh.ray.d) > 0.0f ? bs.exterior : bs.interior;
+  }
+  return true;
+}
+
 // sample_emitter_twostates (plugins/twostatevolpath.py:387-479): emitter
 // direction without visibility, then ratio tracking of both transmittances.
 // GRAD = true replays the same random numbers and scatters
@@ -393,120 +531,18 @@ __device__ inline void nee_twostates(const SceneView &S, V3 ref_p, V3 ref_n, boo
                                      LocalCounters &cnt, float gscale, const GradTargets *G,
                                      NeeRecord *rec = nullptr, float *hom_acc = nullptr,
                                      float *fwd_sum = nullptr) {
-  const float u1 = rng.next_1d();
-  const float u2 = rng.next_1d();
-  const V3 ev = sample_emitter_direction(S, ref_p, u1, u2, ds);
+  Shadow<NS> sh;
+  const V3 ev = shadow_begin<NS>(S, ref_p, ref_n, ref_surface, medium, rng, ds, sh);
 #pragma unroll
   for (int k = 0; k < NS; ++k) em[k] = v3(0.f);
   if (ds.pdf == 0.0f) return;
-  Ray ray;
-  ray.o = ref_surface ? offset_point(ref_p, ref_n, ds.d) : ref_p;
-  ray.d = ds.d;
-  float total = 0.0f;
-  bool needs_int = true;
   SurfInt si;
   si.valid = false;
   si.t = kInf;
-  float T[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) T[k] = 1.0f;
-  while (true) {
-    const float remaining = ds.dist * (1.0f - kShadowEps) - total;
-    ray.maxt = remaining;
-    if (!(remaining > 0.0f)) break;
-    if (needs_int) {
-      Hit h;
-      trace<false>(S, ray, h, stack);
-      cnt.add(kCntTraversals, 1);
-      si = make_si(S, ray, h);
-      needs_int = false;
-    }
-    bool active_medium = medium >= 0;
-    bool active_surface = !active_medium;
-    MediumEv<NS> mev;
-    mev.valid = false;
-    mev.t = kInf;
-    float trm[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) trm[k] = 1.0f;
-    if (active_medium) {
-      const float u = rng.next_1d();
-      sample_medium<NS, F>(S, medium, ray, u, mev);
-      cnt.add(kCntNeeSteps, 1);
-      if (si.t < mev.t) {
-        mev.valid = false;
-        mev.t = kInf;
-      }
-      if ((F & kFeatHomog) && S.media[medium].homogeneous) {
-        const float seg = fminf(fminf(remaining, si.t), si.t) - mev.mint;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) trm[k] = expf(-(seg * mev.st[k]));
-        mev.valid = false;
-        mev.t = kInf;
-        if (rec) rec->hseg += seg;
-        if (GRAD && hom_acc && G->g[S.media[medium].sigma_param]) *hom_acc += -gscale * S.media[medium].scale * seg;
-        if (GRAD && fwd_sum)
-          if (const float *tv = G->g[S.media[medium].sigma_param]) *fwd_sum += -S.media[medium].scale * seg * tv[0];
-      }
-    }
-    const bool escaped = active_medium && !mev.valid;
-    active_medium = active_medium && mev.valid;
-    if (active_medium) {
-      ray.o = mev.p;
-      si.t = si.t - mev.t;
-#pragma unroll
-      for (int k = 0; k < NS; ++k) trm[k] = mev.maj != 0.0f ? mev.sn[k] / mev.maj : mev.sn[k];
-      if (GRAD && (fwd_sum || gscale != 0.0f)) {
-        const MediumInfo &M = S.media[medium];
-        float *g = G->g[M.sigma_param];
-        if (g && mev.sn[NS - 1] != 0.0f) {
-          if (fwd_sum) {
-            *fwd_sum += -M.scale / mev.sn[NS - 1] * grid_eval1(g, mev.foot);
-          } else {
-            scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
-            cnt.add(kCntGradAtomics, 8);
-          }
-        }
-      }
-      if (rec) {
-        unsigned cell;
-        if (rec->n < kNeeRecords && pack_cell(mev.foot, medium, cell)) {
-          const float sn = mev.sn[NS - 1];
-          rec->e[rec->n] = make_float4(mev.foot.tx, mev.foot.ty, mev.foot.tz,
-                                       sn != 0.0f ? -S.media[medium].scale / sn : 0.0f);
-          rec->cell[rec->n] = cell;
-          rec->n++;
-        } else {
-          rec->n = kNeeRecords + 1;
-        }
-      }
-    }
-    active_surface = (active_surface || escaped) && si.valid && !active_medium;
-    if (active_surface) {
-      const float b = (!(F & kFeatSurfaces) || null_bsdf(S, si.shape)) ? 1.0f : 0.0f;
-#pragma unroll
-      for (int k = 0; k < NS; ++k) trm[k] = trm[k] * b;
-    }
-    bool any_t = false;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      T[k] *= trm[k];
-      any_t = any_t || T[k] != 0.0f;
-    }
-    if (active_surface) {
-      ray = spawn_ray(si, ray.d);
-      needs_int = true;
-    }
-    const bool active = (active_medium || active_surface) && any_t;
-    if (!active) break;
-    total += active_medium ? mev.t : si.t;
-    if (active_surface) {
-      const ShapeInfo sh = S.shapes[si.shape];
-      if (sh.interior != sh.exterior) medium = dot(si.n, ray.d) > 0.0f ? sh.exterior : sh.interior;
-    }
+  while (shadow_step<NS, GRAD, F>(S, sh, si, ds.dist, rng, stack, cnt, gscale, G, rec, hom_acc, fwd_sum)) {
   }
 #pragma unroll
-  for (int k = 0; k < NS; ++k) em[k] = ev * T[k];
+  for (int k = 0; k < NS; ++k) em[k] = ev * sh.T[k];
 }
 
 struct AdjState {
@@ -544,6 +580,28 @@ struct NeeWeights {
   float cs[kNeeSlots];
 };
 struct NoNeeWeights {};
+
+// Deferred NEE (renders of scenes without surfaces, nee_start): a
+// wavefront steps only its shadow rays while more than this many of its
+// lanes track one, then one step beside the lanes on their paths (64:
+// never). Measured on the 64^3 volume workload against tracking each shadow
+// ray at once: the single-state render 20-25 % faster with 16 (8 and 32:
+// less), the two-state render ~9 % faster with 64 (any hold below 64 costs
+// it registers in the stepping loop and makes it slower than at once); the
+// adjoint, whose path code is heavier, was slower with every hold (0-64),
+// so it tracks each shadow ray at once.
+constexpr int kNeeHoldPlain = 16;
+constexpr int kNeeHoldTwoState = 64;
+
+// A deferred NEE in flight: the shadow ray and the emitter sample's
+// distance, pdf and value / pdf.
+template <int NS>
+struct NeeFlight {
+  Shadow<NS> sh;
+  float dist, pdf;
+  V3 ev;
+};
+struct NoNeeFlight {};
 
 // One-pass (taped) adjoint: for kernels whose media have a majorant
 // supergrid and whose only medium gradient is the density's. Records live in
@@ -617,6 +675,12 @@ struct VolPath {
   std::conditional_t<kTape, PathTape, NoTape> tape;
   static constexpr bool kNeeInline = ADJ && !FWD && !kTape;
   std::conditional_t<kNeeInline, NeeWeights, NoNeeWeights> neew;
+  // Renders of scenes without surfaces defer the ratio tracking of NEE
+  // shadow rays to the following bounce() calls (nee_start).
+  static constexpr bool kDeferNee = !ADJ && !(F & kFeatSurfaces);
+  static constexpr int kNeeHold = NS == 1 ? kNeeHoldPlain : kNeeHoldTwoState;
+  std::conditional_t<kDeferNee, NeeFlight<NS>, NoNeeFlight> nf;
+  bool in_nee;
   std::conditional_t<ADJ && (F & kFeatHomog) != 0, HomSum, NoHomSum> hom;
 
   // Zeroes the homogeneous-medium sums (once per lane, before its samples).
@@ -649,6 +713,7 @@ struct VolPath {
     medium = -1;
     replay = ADJ && is_replay;
     nee_grad = false;
+    in_nee = false;
     (void)rng.next_1d();  // colour channel for free-flight sampling (:171-173)
     cnt.add(kCntPaths, 1);
   }
@@ -699,9 +764,140 @@ struct VolPath {
     return true;
   }
 
+  // The contribution c[k] = thr nw mis em of an emitter sample whose shadow
+  // ray has been tracked (em: the emitter's value / pdf times the
+  // transmittance; nw, nee_pdf: the BSDF or phase function towards it),
+  // added to L (:331-344), and its adjoint or forward-mode terms. tsum, cs
+  // (have_cs) and rec are what the tracking gathered for them; rng0 is the
+  // random state at the NEE's start, with which a shadow ray is tracked again
+  // when its gradient could not be taken while tracking.
+  __device__ void nee_account(const SceneView &S, const LdsStack &stack, LocalCounters &cnt, const GradTargets *G,
+                              const V3 em[NS], const V3 nw[NS], float nee_pdf, float ds_pdf, float tsum,
+                              bool have_cs, uint64_t rng0, V3 ref_p, V3 ref_n, bool ref_surface, NeeRecord *rec) {
+    const float mis_em = mis_weight(ds_pdf, nee_pdf);
+    V3 c[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      c[k] = thr[k] * nw[k] * mis_em * em[k];
+      L[k] += c[k];
+    }
+    if constexpr (FWD) {
+      // the shadow ray's d(log T)/d(theta) . tangent times this NEE
+      // contribution
+      if (nee_grad && any_nonzero(c[0])) adj.dI += c[0] * tsum;
+    }
+    if constexpr (kNeeInline) {
+      if (!replay) {
+        if (neew.k < kNeeSlots) neew.cs[neew.k] = dot(adj.dL, c[0]);
+      } else if (!have_cs) {
+        // past the slots: track the shadow ray again, now with its weight
+        const float cs = dot(adj.dL, c[0]);
+        if (cs != 0.0f && any_nonzero(c[0])) {
+          Pcg32 r2 = rng;
+          r2.state = rng0;
+          V3 em2[NS];
+          DirSample ds2;
+          float *hacc = nullptr;
+          if constexpr ((F & kFeatHomog) != 0) hacc = &hom.s;
+          nee_twostates<NS, true, F>(S, ref_p, ref_n, ref_surface, medium, r2, em2, ds2, stack, cnt, cs, G, nullptr,
+                                     hacc);
+        }
+      }
+      ++neew.k;
+    }
+    if constexpr (kTape) {
+      const float cs = dot(adj.dL, c[0]);
+      if (nee_grad && cs != 0.0f && any_nonzero(c[0])) {
+        if (rec->n <= kNeeRecords) {
+          if constexpr ((F & kFeatHomog) != 0) {
+            if (rec->hseg != 0.0f) hom.s += -cs * S.media[S.hom_medium].scale * rec->hseg;
+          }
+          // d(log T)/d(density) at each recorded shadow-ray collision
+          RecMedium rm;
+          for (int j = 0; j < rec->n; ++j) {
+            const float4 e = rec->e[j];
+            const unsigned cell = rec->cell[j];
+            rm.select(S, G, cell_medium(cell));
+            if (rm.g && e.w != 0.0f) {
+              scatter_grid(rm.g, rm.foot(e, cell), cs * e.w);
+              cnt.add(kCntGradAtomics, 8);
+            }
+          }
+        } else {
+          Pcg32 r2 = rng;
+          r2.state = rng0;
+          V3 em2[NS];
+          DirSample ds2;
+          float *hacc = nullptr;
+          if constexpr ((F & kFeatHomog) != 0) hacc = &hom.s;
+          nee_twostates<NS, true, F>(S, ref_p, ref_n, ref_surface, medium, r2, em2, ds2, stack, cnt, cs, G, nullptr,
+                                     hacc);
+        }
+      }
+    }
+    if (ADJ && replay) adj.Lrem -= c[0];
+  }
+
+  // Deferred NEE: the emitter sample and shadow ray of a medium NEE at
+  // ref_p. Its ratio tracking then advances one step per bounce() call
+  // (lanes of a wavefront in a NEE and lanes on their paths advance
+  // together, instead of the wavefront waiting for its longest shadow ray),
+  // drawing the same random numbers in the same order as tracking it at
+  // once; the path resumes once the NEE is accounted. The path's surface
+  // record `si` is free until then (a real scattering event traces the new
+  // direction again) and holds the shadow ray's.
+  __device__ void nee_start(const SceneView &S, V3 ref_p, const LdsStack &stack, LocalCounters &cnt,
+                            const GradTargets *G) {
+    if constexpr (kDeferNee) {
+      DirSample ds;
+      nf.ev = shadow_begin<NS>(S, ref_p, v3(0.f), false, medium, rng, ds, nf.sh);
+      nf.dist = ds.dist;
+      nf.pdf = ds.pdf;
+      si.valid = false;
+      si.t = kInf;
+      if (ds.pdf == 0.0f)
+        nee_finish(S, stack, cnt, G);
+      else
+        in_nee = true;
+    }
+  }
+
+  // Deferred NEE: one ratio-tracking step; accounts the NEE once the shadow
+  // ray is done.
+  __device__ void nee_advance(const SceneView &S, const LdsStack &stack, LocalCounters &cnt, const GradTargets *G) {
+    if constexpr (kDeferNee) {
+      if (!shadow_step<NS, false, F>(S, nf.sh, si, nf.dist, rng, stack, cnt, 0.0f, G, nullptr, nullptr, nullptr))
+        nee_finish(S, stack, cnt, G);
+    }
+  }
+
+  __device__ void nee_finish(const SceneView &S, const LdsStack &stack, LocalCounters &cnt, const GradTargets *G) {
+    if constexpr (kDeferNee) {
+      in_nee = false;
+      V3 em[NS], nw[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        em[k] = nf.pdf == 0.0f ? v3(0.f) : nf.ev * nf.sh.T[k];
+        nw[k] = v3(kInv4Pi);
+      }
+      nee_account(S, stack, cnt, G, em, nw, kInv4Pi, nf.pdf, 0.0f, false, 0, ray.o, v3(0.f), false, nullptr);
+    }
+  }
+
   // One bounce; false once the path has ended.
   __device__ bool bounce(const SceneView &S, const IntegArgs &I, const LdsStack &stack, LocalCounters &cnt,
                          const GradTargets *G) {
+    if constexpr (kDeferNee) {
+      // the wavefront steps its shadow rays alone while more than kNeeHold
+      // of its lanes track one, then one step beside the lanes on paths
+      while (__popcll(__ballot(in_nee)) > kNeeHold) {
+        if (in_nee) nee_advance(S, stack, cnt, G);
+      }
+      if (in_nee) {
+        nee_advance(S, stack, cnt, G);
+        return true;
+      }
+    }
     const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
     // ---- Russian roulette (:182-188)
     bool nonzero = false;
@@ -968,109 +1164,55 @@ struct VolPath {
     const bool active_e_medium = act_scatter && sample_emitters;
     if (active_e_surface || active_e_medium) {
       const V3 ref_p = active_e_medium ? mev.p : si.p;
-      V3 em[NS];
-      DirSample ds;
-      const Pcg32 rng_nee = rng;
-      // Untaped adjoint, replay pass: this NEE's dLoss/dL . c from the primal
-      // pass, with which the tracking below scatters the transmittance
-      // gradient.
-      float cs_in = 0.0f;
-      bool have_cs = false;
-      if constexpr (kNeeInline) {
-        if (replay && neew.k < kNeeSlots) {
-          cs_in = neew.cs[neew.k];
-          have_cs = true;
-        }
-      }
-      float tsum = 0.0f;  // forward mode: d(log T)/d(theta) . tangent along the shadow ray
-      NeeRecord rec;      // taped adjoint: the shadow ray's collisions
-      if constexpr (kTape) {
-        rec.n = 0;
-        rec.hseg = 0.0f;
-      }
-      if constexpr (ADJ) {
-        // one instantiation for both passes, so that they compute the same
-        // transmittance from the same random numbers
-        float *hacc = nullptr;
-        if constexpr (kNeeInline && (F & kFeatHomog) != 0) hacc = have_cs ? &hom.s : nullptr;
-        nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, cs_in, G,
-                                   kTape && nee_grad ? &rec : nullptr, hacc, FWD && nee_grad ? &tsum : nullptr);
+      if constexpr (kDeferNee) {
+        nee_start(S, ref_p, stack, cnt, G);  // no surfaces: a medium NEE
       } else {
-        nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G);
-      }
-      V3 nw[NS];
-      float nee_pdf;
-      if (active_e_surface) {
-        const V3 wo = fr.to_local(ds.d);
-  #pragma unroll
-        for (int k = 0; k < NS; ++k)
-          nw[k] = diffuse_eval(diffuse_reflectance(S, bsdf, state_index<NS>(k), si.u, si.v), wi, wo);
-        nee_pdf = diffuse_pdf(wi, wo);
-      } else {
-  #pragma unroll
-        for (int k = 0; k < NS; ++k) nw[k] = v3(kInv4Pi);
-        nee_pdf = kInv4Pi;
-      }
-      const float mis_em = mis_weight(ds.pdf, nee_pdf);
-      V3 c[NS];
-  #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        c[k] = thr[k] * nw[k] * mis_em * em[k];
-        L[k] += c[k];
-      }
-      if constexpr (FWD) {
-        // the shadow ray's d(log T)/d(theta) . tangent times this NEE
-        // contribution
-        if (nee_grad && any_nonzero(c[0])) adj.dI += c[0] * tsum;
-      }
-      if constexpr (kNeeInline) {
-        if (!replay) {
-          if (neew.k < kNeeSlots) neew.cs[neew.k] = dot(adj.dL, c[0]);
-        } else if (!have_cs) {
-          // past the slots: track the shadow ray again, now with its weight
-          const float cs = dot(adj.dL, c[0]);
-          if (cs != 0.0f && any_nonzero(c[0])) {
-            Pcg32 r2 = rng_nee;
-            V3 em2[NS];
-            DirSample ds2;
-            float *hacc = nullptr;
-            if constexpr ((F & kFeatHomog) != 0) hacc = &hom.s;
-            nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G,
-                                       nullptr, hacc);
+        const uint64_t rng0 = rng.state;
+        // Untaped adjoint, replay pass: this NEE's dLoss/dL . c from the
+        // primal pass, with which the tracking below scatters the
+        // transmittance gradient.
+        float cs_in = 0.0f;
+        bool have_cs = false;
+        if constexpr (kNeeInline) {
+          if (replay && neew.k < kNeeSlots) {
+            cs_in = neew.cs[neew.k];
+            have_cs = true;
           }
         }
-        ++neew.k;
-      }
-      if constexpr (kTape) {
-        const float cs = dot(adj.dL, c[0]);
-        if (nee_grad && cs != 0.0f && any_nonzero(c[0])) {
-          if (rec.n <= kNeeRecords) {
-            if constexpr ((F & kFeatHomog) != 0) {
-              if (rec.hseg != 0.0f) hom.s += -cs * S.media[S.hom_medium].scale * rec.hseg;
-            }
-            // d(log T)/d(density) at each recorded shadow-ray collision
-            RecMedium rm;
-            for (int j = 0; j < rec.n; ++j) {
-              const float4 e = rec.e[j];
-              const unsigned cell = rec.cell[j];
-              rm.select(S, G, cell_medium(cell));
-              if (rm.g && e.w != 0.0f) {
-                scatter_grid(rm.g, rm.foot(e, cell), cs * e.w);
-                cnt.add(kCntGradAtomics, 8);
-              }
-            }
-          } else {
-            Pcg32 r2 = rng_nee;
-            V3 em2[NS];
-            DirSample ds2;
-            float *hacc = nullptr;
-            if constexpr ((F & kFeatHomog) != 0) hacc = &hom.s;
-            nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, r2, em2, ds2, stack, cnt, cs, G,
-                                       nullptr, hacc);
-          }
+        float tsum = 0.0f;  // forward mode: d(log T)/d(theta) . tangent along the shadow ray
+        NeeRecord rec;      // taped adjoint: the shadow ray's collisions
+        if constexpr (kTape) {
+          rec.n = 0;
+          rec.hseg = 0.0f;
         }
+        V3 em[NS];
+        DirSample ds;
+        if constexpr (ADJ) {
+          // one instantiation for both passes, so that they compute the same
+          // transmittance from the same random numbers
+          float *hacc = nullptr;
+          if constexpr (kNeeInline && (F & kFeatHomog) != 0) hacc = have_cs ? &hom.s : nullptr;
+          nee_twostates<NS, true, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, cs_in, G,
+                                     kTape && nee_grad ? &rec : nullptr, hacc, FWD && nee_grad ? &tsum : nullptr);
+        } else {
+          nee_twostates<NS, false, F>(S, ref_p, si.n, active_e_surface, medium, rng, em, ds, stack, cnt, 0.f, G);
+        }
+        V3 nw[NS];
+        float nee_pdf;
+        if (active_e_surface) {
+          const V3 wo = fr.to_local(ds.d);
+  #pragma unroll
+          for (int k = 0; k < NS; ++k)
+            nw[k] = diffuse_eval(diffuse_reflectance(S, bsdf, state_index<NS>(k), si.u, si.v), wi, wo);
+          nee_pdf = diffuse_pdf(wi, wo);
+        } else {
+  #pragma unroll
+          for (int k = 0; k < NS; ++k) nw[k] = v3(kInv4Pi);
+          nee_pdf = kInv4Pi;
+        }
+        nee_account(S, stack, cnt, G, em, nw, nee_pdf, ds.pdf, tsum, have_cs, rng0, ref_p, si.n, active_e_surface,
+                    &rec);
       }
-      if (ADJ && replay) adj.Lrem -= c[0];
     }
 
     // ---- BSDF sampling (:347-380)
