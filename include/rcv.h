@@ -348,8 +348,9 @@ int rcv_batch_add_job(rcv_batch *batch, const rcv_batch_job *job, int32_t *out_j
  * bound to param_ring[0] (OLD) and param_ring[1] (NEW) beforehand. */
 int rcv_batch_commit(rcv_batch *batch, void *stream);
 /* Launch shape of the persistent kernels: threads per block (64..512, a
- * multiple of 64) and work units taken per queue grab (64..8192, a multiple
- * of 64); 0 = defaults. Invalidates a captured graph. */
+ * multiple of 64, and at most what the device's LDS per block holds at 160 B
+ * per thread: RCV_E_INVALID otherwise) and work units taken per queue grab
+ * (64..8192, a multiple of 64); 0 = defaults. Invalidates a captured graph. */
 int rcv_batch_set_launch(rcv_batch *batch, int32_t block_size, int32_t queue_grab);
 /* One iteration of every job on `stream` (must not be the legacy default
  * stream when use_graph != 0: the first call captures a hipGraph, later calls

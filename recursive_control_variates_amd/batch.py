@@ -255,14 +255,19 @@ class BatchOptimizer:
 def autotune(make_specs, blocks=(64, 128, 256, 512), grabs=(64, 128, 256, 1024), steps=3, device="cuda"):
     """Times ``steps`` batched iterations for every (block size, queue grab)
     pair on a throw-away batch built by ``make_specs()`` and returns the
-    fastest pair with the full table of timings (ms per iteration)."""
+    fastest pair with the full table of timings (ms per iteration). Block
+    sizes the device cannot launch (LDS per block) are left out of the
+    table."""
     table = {}
     opt = BatchOptimizer(make_specs(), device=device, use_graph=False)
     opt.step(1)
     opt.synchronize()
     for b in blocks:
         for g in grabs:
-            opt.set_launch(b, g)
+            try:
+                opt.set_launch(b, g)
+            except N.RcvError:
+                continue
             opt.step(1)
             opt.synchronize()
             t0 = time.perf_counter()

@@ -153,7 +153,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
                                     const int64_t *__restrict__ ends, int n, int64_t total, unsigned *queue,
                                     int grab) {
   extern __shared__ int lds_stack[];
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
   const int lane = threadIdx.x & (kWave - 1);
   while (true) {
     unsigned b = 0;
@@ -166,6 +165,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
       const BatchJobDev &J = jobs[ids[k]];
       if (!J.active) continue;
       const SceneView &S = J.view;
+      const LdsStack stack = lds_of(S, lds_stack);
       const int64_t local = u0 + lane - J.unit_begin;
       const int64_t pixel = local >> J.cl2;
       const int chunk = int(local & ((1 << J.cl2) - 1));
@@ -300,7 +300,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
                                      const int64_t *__restrict__ ends, int n, int64_t total, unsigned *queue,
                                      int grab) {
   extern __shared__ int lds_stack[];
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
   const int lane = threadIdx.x & (kWave - 1);
   while (true) {
     unsigned b = 0;
@@ -313,6 +312,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu
       const BatchJobDev &J = jobs[ids[k]];
       if (!J.active) continue;
       const SceneView &S = J.view;
+      const LdsStack stack = lds_of(S, lds_stack);
       const int64_t local = u0 + lane - J.unit_begin_g;
       const int64_t pixel = local >> J.cl2_g;
       const int chunk = int(local & ((1 << J.cl2_g) - 1));
@@ -657,9 +657,14 @@ extern "C" int rcv_batch_set_launch(rcv_batch *b, int32_t block_size, int32_t qu
               "rcv_batch_set_launch: block size must be a multiple of 64 in [64, 512]");
   RCV_REQUIRE(queue_grab >= 64 && queue_grab <= 8192 && queue_grab % 64 == 0,
               "rcv_batch_set_launch: queue grab must be a multiple of 64 in [64, 8192]");
+  const size_t lds = size_t(kLdsWordsPerThread) * block_size * sizeof(int);
+  int dev = 0, max_lds = 0;
+  RCV_TRY_HIP(hipGetDevice(&dev), "hipGetDevice");
+  RCV_TRY_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "LDS per block");
+  RCV_REQUIRE(lds <= size_t(max_lds), "rcv_batch_set_launch: block size %d needs %zu B of LDS per block, above the %d B "
+              "this device allows", block_size, lds, max_lds);
   b->block = block_size;
   b->grab = queue_grab;
-  const size_t lds = size_t(kStackSize) * block_size * sizeof(int);
   int g1 = 0, g2 = 0;
   int rc = RCV_OK;
   with_features(b->feat, [&](auto f) {
@@ -683,7 +688,7 @@ extern "C" int rcv_batch_set_launch(rcv_batch *b, int32_t block_size, int32_t qu
 
 static int enqueue_step(rcv_batch *b, hipStream_t st) {
   const int n = int(b->host_jobs.size());
-  const size_t lds = size_t(kStackSize) * b->block * sizeof(int);
+  const size_t lds = size_t(kLdsWordsPerThread) * b->block * sizeof(int);
   hipLaunchKernelGGL(batch_rotate_kernel, dim3(ceil_div(std::max(n, 6), 64)), dim3(64), 0, st, b->d_jobs, n,
                      b->d_queues);
   if (!b->sg_ids.empty()) {

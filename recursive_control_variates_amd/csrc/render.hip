@@ -9,8 +9,9 @@
 // twostatepath.py:168-177) is written once, coalesced. The random stream of a
 // camera sample depends only on (seed, pixel * spp + sample), so the launch
 // shape changes results only through the float summation order of a pixel.
-// The BVH traversal stack of each thread lives in LDS (kStackSize ints,
-// strided by the block size).
+// The BVH traversal stack of each thread lives in LDS (as deep as the
+// scene's BVH, strided by the block size), followed by per-lane slots (the
+// adjoint's NEE weights): kLdsWordsPerThread words per thread in all.
 
 #include <algorithm>
 #include <map>
@@ -35,8 +36,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
                                                         int spp, int spt, int chunk_log2, float *img_old,
                                                         float *img_new) {
-  extern __shared__ int lds_stack[];  // kStackSize ints per thread
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
+  const LdsStack stack = lds_of(S, lds_stack);
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
   const int64_t n_pix = int64_t(film_w) * sen.film_h;
@@ -127,8 +128,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
                                                          int spp, int spt, int chunk_log2,
                                                          const float *__restrict__ grad_img, GradTargets G) {
-  extern __shared__ int lds_stack[];  // kStackSize ints per thread
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
+  const LdsStack stack = lds_of(S, lds_stack);
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
   const int64_t n_pix = int64_t(film_w) * sen.film_h;
@@ -162,8 +163,8 @@ template <int INTEG, int F>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void forward_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int spt, int chunk_log2, GradTargets T,
     float *__restrict__ dimg) {
-  extern __shared__ int lds_stack[];  // kStackSize ints per thread
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
+  const LdsStack stack = lds_of(S, lds_stack);
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
   const int64_t n_pix = int64_t(film_w) * sen.film_h;
@@ -218,8 +219,8 @@ template <int F>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH>))) void
 adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2,
                           const float *__restrict__ grad_img, GradTargets G, unsigned *queue, unsigned total_units) {
-  extern __shared__ int lds_stack[];  // kStackSize ints per thread
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
+  const LdsStack stack = lds_of(S, lds_stack);
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
   LocalCounters cnt{S.counters};
@@ -278,8 +279,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
 adjoint_sample_persistent_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2, const float *__restrict__ grad_img,
     GradTargets G, unsigned *queue, unsigned total_units) {
-  extern __shared__ int lds_stack[];  // kStackSize ints per thread
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
+  const LdsStack stack = lds_of(S, lds_stack);
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
   LocalCounters cnt{S.counters};
@@ -323,8 +324,8 @@ template <int NS, int F>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH>))) void
 render_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2,
                          float *__restrict__ partial, unsigned *queue, unsigned total_units) {
-  extern __shared__ int lds_stack[];  // kStackSize ints per thread
-  const LdsStack stack{lds_stack + threadIdx.x, int(blockDim.x)};
+  extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
+  const LdsStack stack = lds_of(S, lds_stack);
   const SensorInfo &sen = S.sensors[sensor];
   const int film_w = sen.film_w;
   LocalCounters cnt{S.counters};
@@ -413,7 +414,7 @@ __global__ void zero_counters_kernel(unsigned long long *c) {
 
 }  // namespace
 
-static size_t stack_bytes(int block) { return size_t(kStackSize) * block * sizeof(int); }
+static size_t stack_bytes(int block) { return size_t(kLdsWordsPerThread) * block * sizeof(int); }
 
 // Blocks of `kern` the current device holds at once (compute units x
 // resident blocks per unit), cached per (device, kernel, block size).

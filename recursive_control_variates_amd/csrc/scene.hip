@@ -58,6 +58,7 @@ struct Builder {
   std::vector<BuildPrim> &prims;
   std::vector<BVHNode> nodes;
 
+  int max_depth = 0;  // deepest leaf (root: 0), the most entries a traversal stack holds
   explicit Builder(std::vector<BuildPrim> &p) : prims(p) {}
 
   void make_leaf(int node, int begin, int end, const AABB &box) {
@@ -82,6 +83,7 @@ struct Builder {
         // Depth limit reached: the leaf keeps all remaining primitives.
       }
       make_leaf(node, begin, end, box);
+      max_depth = std::max(max_depth, depth);
       return;
     }
     // Binned SAH over the centroid bounds.
@@ -259,6 +261,7 @@ int Scene::upload(hipStream_t stream) {
   Builder b(bp);
   b.nodes.push_back(BVHNode{});
   b.build(0, 0, int(bp.size()), 0);
+  const int bvh_depth = b.max_depth;
   const int n_slots = int(bp.size());
 
   std::vector<float4> tris(3 * n_slots, make_float4(0, 0, 0, 0));
@@ -408,6 +411,9 @@ int Scene::upload(hipStream_t stream) {
   v.handle_null = 0;
   v.nee_homog = 0;
   v.hom_medium = -1;
+  // a traversal pushes at most one node per level above the current one
+  v.lds_stack = std::max(1, bvh_depth);
+  v.lds_slots = std::min(kMaxLdsSlots, kLdsWordsPerThread - v.lds_stack);
   for (int i = 0; i < int(media.size()); ++i) {
     if (media[i].homogeneous) {
       v.nee_homog = 1;

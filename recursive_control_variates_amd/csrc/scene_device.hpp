@@ -195,16 +195,27 @@ __device__ inline bool hit_box(const BVHNode &n, V3 o, V3 inv_d, float tmax, flo
   return t0 <= t1;
 }
 
-// Per-thread traversal stack: kStackSize ints in LDS, strided by the block
-// size so that the 64 lanes of a wave hit 64 distinct banks.
+// Per-thread traversal stack in LDS, strided by the block size so that the
+// 64 lanes of a wave hit 64 distinct banks, and after it the thread's
+// `nslots` words of per-lane storage (slot(j), same stride).
 struct LdsStack {
   int *base;
   int stride;
+  int *slots;
+  int nslots;
   __device__ int &operator[](int i) const { return base[i * stride]; }
+  __device__ int &slot(int j) const { return slots[j * stride]; }
 };
 
+// The thread's view of a launch's dynamic LDS (kLdsWordsPerThread words per
+// thread) for scene S: its stack, then its slots.
+__device__ inline LdsStack lds_of(const SceneView &S, int *lds) {
+  const int tid = int(threadIdx.x), stride = int(blockDim.x);
+  return LdsStack{lds + tid, stride, lds + S.lds_stack * stride + tid, S.lds_slots};
+}
+
 // Closest hit (any_hit = false) or occlusion test (any_hit = true) against
-// [0, r.maxt). Stack depth never exceeds the BVH depth (< kStackSize).
+// [0, r.maxt). Stack depth never exceeds the BVH depth (SceneView::lds_stack).
 template <bool ANY_HIT>
 __device__ inline bool trace(const SceneView &S, const Ray &r, Hit &h, const LdsStack &stack) {
   const V3 inv_d = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);

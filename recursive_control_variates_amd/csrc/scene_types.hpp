@@ -18,6 +18,14 @@ constexpr int kMaxParams = 8;
 constexpr int kMaxViews = 8;
 constexpr int kMaxBvhDepth = 30;
 constexpr int kStackSize = 32;
+// Per-thread LDS of the trace kernels, in 4-byte words: the BVH traversal
+// stack (as deep as the scene's BVH, SceneView::lds_stack) and then per-lane
+// slots (the adjoint's NEE weights, SceneView::lds_slots, at most
+// kMaxLdsSlots). 40 words = 160 KB of LDS per CU over the 16 waves of 64
+// lanes that the trace kernels' registers allow (four per SIMD).
+constexpr int kLdsWordsPerThread = 40;
+constexpr int kMaxLdsSlots = 32;
+static_assert(kMaxBvhDepth < kLdsWordsPerThread, "a BVH of the deepest kind leaves no LDS for its stack");
 constexpr int kNumCounters = 8;
 
 enum Counter : int {
@@ -145,6 +153,8 @@ struct SceneView {
   int32_t handle_null;
   int32_t nee_homog;
   int32_t hom_medium;   // the scene's homogeneous medium (at most one) or -1
+  int32_t lds_stack;    // LDS words per thread for the BVH traversal stack (the BVH depth)
+  int32_t lds_slots;    // LDS words per thread after it: kLdsWordsPerThread - lds_stack, at most kMaxLdsSlots
   float bs_center[3];
   float bs_radius;
   ParamBind params[kMaxParams];

@@ -569,29 +569,30 @@ struct PathTape {
 struct NoTape {};
 
 // Untaped adjoint: dLoss/dL . c of each NEE contribution c of the primal
-// pass, in path order. The replay pass reads the value before it tracks the
-// same shadow ray (same random numbers, same collisions), so it scatters the
-// transmittance gradient while tracking: one shadow-ray tracking per pass and
-// nothing recorded per collision. A path with more NEEs than kNeeSlots
-// tracks the later ones' shadow rays again in the replay.
-constexpr int kNeeSlots = 32;
+// pass, in path order, in the lane's LDS slots (LdsStack::slot). The replay
+// pass reads the value before it tracks the same shadow ray (same random
+// numbers, same collisions), so it scatters the transmittance gradient while
+// tracking: one shadow-ray tracking per pass and nothing recorded per
+// collision. A path with more NEEs than slots tracks the later ones' shadow
+// rays again in the replay. (Per-lane scratch instead of LDS cost the 64^3
+// adjoint ~6 ms at a mid state of the optimisation and doubled it at the
+// initial one: each slot's store and load went to memory.)
 struct NeeWeights {
-  int k;
-  float cs[kNeeSlots];
+  int k;  // NEEs of the path so far
 };
 struct NoNeeWeights {};
 
-// Deferred NEE (renders of scenes without surfaces, nee_start): a
-// wavefront steps only its shadow rays while more than this many of its
-// lanes track one, then one step beside the lanes on their paths (64:
-// never). Measured on the 64^3 volume workload against tracking each shadow
-// ray at once: the single-state render 20-25 % faster with 16 (8 and 32:
-// less), the two-state render ~9 % faster with 64 (any hold below 64 costs
-// it registers in the stepping loop and makes it slower than at once); the
-// adjoint, whose path code is heavier, was slower with every hold (0-64),
-// so it tracks each shadow ray at once.
-constexpr int kNeeHoldPlain = 16;
-constexpr int kNeeHoldTwoState = 64;
+// Deferred NEE (single-state renders of scenes without surfaces,
+// nee_start): a wavefront steps only its shadow rays while more than this
+// many of its lanes track one, then one step beside the lanes on their
+// paths. Measured on the 64^3 volume workload against tracking each shadow
+// ray at once: the single-state render 13-27 % faster with 16 (8, 32 and 64:
+// less). The two-state render (twice the lookups and state per lane) was
+// slower deferred over the optimisation (15.3 -> 19.1 ms per launch, the
+// 64-job batch 354 -> 480 ms) and the adjoint (heavier path code, run on
+// every trip) slower with every hold from 0 to 64, so both track each shadow
+// ray at once.
+constexpr int kNeeHold = 16;
 
 // A deferred NEE in flight: the shadow ray and the emitter sample's
 // distance, pdf and value / pdf.
@@ -675,10 +676,9 @@ struct VolPath {
   std::conditional_t<kTape, PathTape, NoTape> tape;
   static constexpr bool kNeeInline = ADJ && !FWD && !kTape;
   std::conditional_t<kNeeInline, NeeWeights, NoNeeWeights> neew;
-  // Renders of scenes without surfaces defer the ratio tracking of NEE
-  // shadow rays to the following bounce() calls (nee_start).
-  static constexpr bool kDeferNee = !ADJ && !(F & kFeatSurfaces);
-  static constexpr int kNeeHold = NS == 1 ? kNeeHoldPlain : kNeeHoldTwoState;
+  // Single-state renders of scenes without surfaces defer the ratio
+  // tracking of NEE shadow rays to the following bounce() calls (nee_start).
+  static constexpr bool kDeferNee = !ADJ && NS == 1 && !(F & kFeatSurfaces);
   std::conditional_t<kDeferNee, NeeFlight<NS>, NoNeeFlight> nf;
   bool in_nee;
   std::conditional_t<ADJ && (F & kFeatHomog) != 0, HomSum, NoHomSum> hom;
@@ -788,7 +788,7 @@ struct VolPath {
     }
     if constexpr (kNeeInline) {
       if (!replay) {
-        if (neew.k < kNeeSlots) neew.cs[neew.k] = dot(adj.dL, c[0]);
+        if (neew.k < stack.nslots) stack.slot(neew.k) = __float_as_int(dot(adj.dL, c[0]));
       } else if (!have_cs) {
         // past the slots: track the shadow ray again, now with its weight
         const float cs = dot(adj.dL, c[0]);
@@ -1174,8 +1174,8 @@ struct VolPath {
         float cs_in = 0.0f;
         bool have_cs = false;
         if constexpr (kNeeInline) {
-          if (replay && neew.k < kNeeSlots) {
-            cs_in = neew.cs[neew.k];
+          if (replay && neew.k < stack.nslots) {
+            cs_in = __int_as_float(stack.slot(neew.k));
             have_cs = true;
           }
         }

@@ -74,6 +74,37 @@ def test_adjoint_with_tape_overflows_matches_oracle_same_samples(cuda):
     assert abs(g_gpu.sum() - g_cpu.sum()) < 0.03 * np.abs(g_cpu).sum()
 
 
+def test_adjoint_with_more_nees_than_slots_matches_oracle_same_samples(cuda):
+    # Without a supergrid the primal pass keeps dLoss/dL . c of each NEE in
+    # the lane's LDS slots (at most 32) and the replay scatters the shadow
+    # ray's gradient while tracking it; NEEs past the slots are tracked a
+    # second time in the replay. A thick, bright medium with paths of up to
+    # 64 scattering events makes most paths go past the slots.
+    announce("adjoint vs oracle fp32 (thick, more NEEs per path than LDS slots)")
+    rng = np.random.default_rng(6)
+    grid = (0.2 + rng.uniform(0, 0.8, (5, 5, 5, 1))).astype(np.float32)
+    w, h, spp, depth = 8, 8, 16, 64
+    sc = rcv.load_dict(volume_box_scene(grid, albedo=0.95, scale=40.0, w=w, h=h, spp=spp))
+    gimg = rng.uniform(0.5, 1.5, (h, w, 3))
+    integ = rcv.load_dict({"type": "prbvolpath", "max_depth": depth, "rr_depth": depth})
+    sc.set_counting(True)
+    integ.render(sc, 0, 17, spp)
+    primal = sc.read_counters()
+    sc.set_counting(False)
+    sc.set_counting(True)
+    g_gpu = _adjoint(sc, integ, gimg, 17, spp)
+    cnt = sc.read_counters()
+    sc.set_counting(False)
+    # the adjoint's primal and replay passes track every shadow ray once each
+    # (the render of the same samples, twice); past the slots, once more
+    assert cnt["nee_steps"] > 2.1 * primal["nee_steps"], (cnt, primal)
+    g_cpu = R.adjoint(sc.desc, depth, depth, 17, spp, gimg, precision=32)
+    scale = np.abs(g_cpu).max()
+    close = np.abs(g_gpu - g_cpu) <= 1e-3 * scale + 1e-3 * np.abs(g_cpu)
+    assert close.mean() > 0.97, close.mean()
+    assert abs(g_gpu.sum() - g_cpu.sum()) < 0.03 * np.abs(g_cpu).sum()
+
+
 def test_adjoint_matches_finite_differences(cuda):
     announce("adjoint vs FD")
     rng, grid, sc = _setup(2)
