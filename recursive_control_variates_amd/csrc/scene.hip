@@ -530,6 +530,9 @@ int Scene::features() const {
   int f = 0;
   for (const ShapeInfo &sh : shapes)
     if (sh.bsdf < 0 || bsdfs[sh.bsdf].type != RCV_BSDF_NULL) f |= kFeatSurfaces;
+  // the kernels with surface code track NEE shadow rays at once, needing no
+  // LDS slots for them: the choice when a deep BVH's stack leaves too few
+  if (view.lds_slots < kNeeFlightWords) f |= kFeatSurfaces;
   for (const MediumInfo &m : media) {
     if (m.homogeneous) {
       f |= kFeatHomog;

@@ -26,6 +26,8 @@ constexpr int kStackSize = 32;
 constexpr int kLdsWordsPerThread = 40;
 constexpr int kMaxLdsSlots = 32;
 static_assert(kMaxBvhDepth < kLdsWordsPerThread, "a BVH of the deepest kind leaves no LDS for its stack");
+// LDS slots a render's deferred NEE keeps its shadow ray in.
+constexpr int kNeeFlightWords = 16;
 constexpr int kNumCounters = 8;
 
 enum Counter : int {

@@ -582,27 +582,52 @@ struct NeeWeights {
 };
 struct NoNeeWeights {};
 
-// Deferred NEE (single-state renders of scenes without surfaces,
-// nee_start): a wavefront steps only its shadow rays while more than this
-// many of its lanes track one, then one step beside the lanes on their
-// paths. Measured on the 64^3 volume workload against tracking each shadow
-// ray at once: the single-state render 13-27 % faster with 16 (8, 32 and 64:
-// less). The two-state render (twice the lookups and state per lane) was
-// slower deferred over the optimisation (15.3 -> 19.1 ms per launch, the
-// 64-job batch 354 -> 480 ms) and the adjoint (heavier path code, run on
-// every trip) slower with every hold from 0 to 64, so both track each shadow
-// ray at once.
+// Deferred NEE (renders of scenes without surfaces, nee_start): after the
+// lanes on their paths have advanced them, a wavefront steps only its shadow
+// rays while more than this many of its lanes track one, then once more with
+// all of them. Measured on the 64^3 volume workload against tracking each
+// shadow ray at once, at mid and final states of the optimisation: the
+// single-state render 25-27 % and the two-state render 23-24 % faster with
+// 16 (8, 32 and 64: less), slower only at a uniform density, where every
+// shadow ray ends at its first collision. The adjoint, whose path code (run
+// on every trip) is heavier than a tracking step, was slower with every hold
+// from 0 to 64 and tracks each shadow ray at once.
 constexpr int kNeeHold = 16;
 
-// A deferred NEE in flight: the shadow ray and the emitter sample's
-// distance, pdf and value / pdf.
+// A deferred NEE in flight lives in the lane's LDS slots (a render kernel
+// uses them for nothing else), so that it holds no registers across bounce()
+// calls: words 0-5 the shadow ray's origin and direction, 6 the distance
+// tracked, 7 its medium, 8 whether its next surface must be traced, 9-10 the
+// transmittances, 11 the emitter sample's distance, 12 its pdf, 13-15 its
+// value / pdf (kNeeFlightWords words). Scenes with fewer slots (a deep BVH)
+// take the kernels that track at once (Scene::features).
+
 template <int NS>
-struct NeeFlight {
-  Shadow<NS> sh;
-  float dist, pdf;
-  V3 ev;
-};
-struct NoNeeFlight {};
+__device__ inline void flight_put(const LdsStack &L, const Shadow<NS> &sh) {
+  L.slot(0) = __float_as_int(sh.ray.o.x);
+  L.slot(1) = __float_as_int(sh.ray.o.y);
+  L.slot(2) = __float_as_int(sh.ray.o.z);
+  L.slot(3) = __float_as_int(sh.ray.d.x);
+  L.slot(4) = __float_as_int(sh.ray.d.y);
+  L.slot(5) = __float_as_int(sh.ray.d.z);
+  L.slot(6) = __float_as_int(sh.total);
+  L.slot(7) = sh.medium;
+  L.slot(8) = sh.needs_int ? 1 : 0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) L.slot(9 + k) = __float_as_int(sh.T[k]);
+}
+
+template <int NS>
+__device__ inline void flight_get(const LdsStack &L, Shadow<NS> &sh) {
+  sh.ray.o = v3(__int_as_float(L.slot(0)), __int_as_float(L.slot(1)), __int_as_float(L.slot(2)));
+  sh.ray.d = v3(__int_as_float(L.slot(3)), __int_as_float(L.slot(4)), __int_as_float(L.slot(5)));
+  sh.ray.maxt = kInf;
+  sh.total = __int_as_float(L.slot(6));
+  sh.medium = L.slot(7);
+  sh.needs_int = L.slot(8) != 0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) sh.T[k] = __int_as_float(L.slot(9 + k));
+}
 
 // One-pass (taped) adjoint: for kernels whose media have a majorant
 // supergrid and whose only medium gradient is the density's. Records live in
@@ -676,10 +701,9 @@ struct VolPath {
   std::conditional_t<kTape, PathTape, NoTape> tape;
   static constexpr bool kNeeInline = ADJ && !FWD && !kTape;
   std::conditional_t<kNeeInline, NeeWeights, NoNeeWeights> neew;
-  // Single-state renders of scenes without surfaces defer the ratio
-  // tracking of NEE shadow rays to the following bounce() calls (nee_start).
-  static constexpr bool kDeferNee = !ADJ && NS == 1 && !(F & kFeatSurfaces);
-  std::conditional_t<kDeferNee, NeeFlight<NS>, NoNeeFlight> nf;
+  // Renders of scenes without surfaces defer the ratio tracking of NEE
+  // shadow rays to the following bounce() calls (nee_start).
+  static constexpr bool kDeferNee = !ADJ && !(F & kFeatSurfaces);
   bool in_nee;
   std::conditional_t<ADJ && (F & kFeatHomog) != 0, HomSum, NoHomSum> hom;
 
@@ -850,15 +874,21 @@ struct VolPath {
                             const GradTargets *G) {
     if constexpr (kDeferNee) {
       DirSample ds;
-      nf.ev = shadow_begin<NS>(S, ref_p, v3(0.f), false, medium, rng, ds, nf.sh);
-      nf.dist = ds.dist;
-      nf.pdf = ds.pdf;
+      Shadow<NS> sh;
+      const V3 ev = shadow_begin<NS>(S, ref_p, v3(0.f), false, medium, rng, ds, sh);
       si.valid = false;
       si.t = kInf;
-      if (ds.pdf == 0.0f)
-        nee_finish(S, stack, cnt, G);
-      else
+      if (ds.pdf == 0.0f) {
+        nee_finish(S, stack, cnt, G, sh, 0.0f, ev);
+      } else {
+        flight_put<NS>(stack, sh);
+        stack.slot(11) = __float_as_int(ds.dist);
+        stack.slot(12) = __float_as_int(ds.pdf);
+        stack.slot(13) = __float_as_int(ev.x);
+        stack.slot(14) = __float_as_int(ev.y);
+        stack.slot(15) = __float_as_int(ev.z);
         in_nee = true;
+      }
     }
   }
 
@@ -866,38 +896,54 @@ struct VolPath {
   // ray is done.
   __device__ void nee_advance(const SceneView &S, const LdsStack &stack, LocalCounters &cnt, const GradTargets *G) {
     if constexpr (kDeferNee) {
-      if (!shadow_step<NS, false, F>(S, nf.sh, si, nf.dist, rng, stack, cnt, 0.0f, G, nullptr, nullptr, nullptr))
-        nee_finish(S, stack, cnt, G);
+      Shadow<NS> sh;
+      flight_get<NS>(stack, sh);
+      const float dist = __int_as_float(stack.slot(11));
+      if (shadow_step<NS, false, F>(S, sh, si, dist, rng, stack, cnt, 0.0f, G, nullptr, nullptr, nullptr)) {
+        flight_put<NS>(stack, sh);
+      } else {
+        const V3 ev = v3(__int_as_float(stack.slot(13)), __int_as_float(stack.slot(14)),
+                         __int_as_float(stack.slot(15)));
+        nee_finish(S, stack, cnt, G, sh, __int_as_float(stack.slot(12)), ev);
+      }
     }
   }
 
-  __device__ void nee_finish(const SceneView &S, const LdsStack &stack, LocalCounters &cnt, const GradTargets *G) {
+  __device__ void nee_finish(const SceneView &S, const LdsStack &stack, LocalCounters &cnt, const GradTargets *G,
+                             const Shadow<NS> &sh, float pdf, V3 ev) {
     if constexpr (kDeferNee) {
       in_nee = false;
       V3 em[NS], nw[NS];
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        em[k] = nf.pdf == 0.0f ? v3(0.f) : nf.ev * nf.sh.T[k];
+        em[k] = pdf == 0.0f ? v3(0.f) : ev * sh.T[k];
         nw[k] = v3(kInv4Pi);
       }
-      nee_account(S, stack, cnt, G, em, nw, kInv4Pi, nf.pdf, 0.0f, false, 0, ray.o, v3(0.f), false, nullptr);
+      nee_account(S, stack, cnt, G, em, nw, kInv4Pi, pdf, 0.0f, false, 0, ray.o, v3(0.f), false, nullptr);
     }
   }
 
-  // One bounce; false once the path has ended.
+  // One bounce; false once the path has ended. With deferred NEE, the lanes
+  // on their paths advance them first (a lane may start a NEE there), then
+  // the lanes with a shadow ray in flight step it: alone while more than
+  // kNeeHold of them track one, then once more together, so a shadow ray
+  // that ends at its first collision costs no extra trip.
   __device__ bool bounce(const SceneView &S, const IntegArgs &I, const LdsStack &stack, LocalCounters &cnt,
                          const GradTargets *G) {
+    bool live = true;
+    if (!kDeferNee || !in_nee) live = path_step(S, I, stack, cnt, G);
     if constexpr (kDeferNee) {
-      // the wavefront steps its shadow rays alone while more than kNeeHold
-      // of its lanes track one, then one step beside the lanes on paths
       while (__popcll(__ballot(in_nee)) > kNeeHold) {
         if (in_nee) nee_advance(S, stack, cnt, G);
       }
-      if (in_nee) {
-        nee_advance(S, stack, cnt, G);
-        return true;
-      }
+      if (in_nee) nee_advance(S, stack, cnt, G);
     }
+    return live;
+  }
+
+  // The body of the reference's loop for one bounce of the path.
+  __device__ bool path_step(const SceneView &S, const IntegArgs &I, const LdsStack &stack, LocalCounters &cnt,
+                            const GradTargets *G) {
     const int max_depth = I.max_depth < 0 ? 0x7fffffff : I.max_depth;
     // ---- Russian roulette (:182-188)
     bool nonzero = false;
