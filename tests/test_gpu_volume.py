@@ -43,6 +43,56 @@ def test_two_state_matches_oracle_sample_by_sample(cuda):
         assert abs(g.mean() - o.mean()) < 1e-3 * abs(o.mean()) + 1e-5
 
 
+def test_null_triangles_and_deep_bvh_match_oracle(cuda):
+    # 1500 small null-BSDF triangles inside the medium: a BVH several levels
+    # deep (the traversal stack in LDS is sized to the scene's BVH, the NEE
+    # shadow ray in flight sits after it), and surfaces that paths and the
+    # deferred shadow rays cross without effect, in the two-state render, the
+    # plain render and the adjoint.
+    announce("volume with 1500 null triangles (deep BVH) vs oracle fp32")
+    ga, gb = _grid(11), _grid(12)
+    d = volume_box_scene(ga, scale=6.0, w=12, h=10, spp=8)
+    rng = np.random.default_rng(13)
+    centres = rng.uniform(0.1, 0.9, (1500, 1, 3))
+    tris = centres + rng.uniform(-0.03, 0.03, (1500, 3, 3))
+    d["veil"] = {"type": "mesh", "vertices": tris.reshape(-1, 3), "faces": np.arange(4500).reshape(-1, 3),
+                 "bsdf": {"type": "null"}}
+    sc = rcv.load_dict(d)
+    p = rcv.traverse(sc)
+    key = "box.interior_medium.sigma_t.data"
+    p[key] = torch.from_numpy(gb).cuda()
+    p.update()                       # old = ga, new = gb
+    ts = rcv.load_dict({"type": "twostateprbvolpath", "max_depth": 8, "rr_depth": 8})
+    sc.set_counting(True)
+    H, F = ts.render_twostates(sc, 0, 321, 8)
+    cnt = sc.read_counters()
+    sc.set_counting(False)
+    assert cnt["traversals"] > 0 and cnt["nee_steps"] > 0, cnt
+    Ho, Fo = R.render(sc.desc, VOL, 8, 8, 321, 8, params_old=[ga], params_new=[gb], precision=32)
+    for g, o in ((H, Ho), (F, Fo)):
+        g = g.cpu().numpy()
+        assert _close_fraction(g, o) > 0.97
+        assert abs(g.mean() - o.mean()) < 2e-3 * abs(o.mean()) + 1e-5
+    # the plain render of the NEW state: the oracle's render with both states
+    # at the NEW one
+    plain = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    P = plain.render(sc, 0, 321, 8).cpu().numpy()
+    _, Po = R.render(sc.desc, VOL, 8, 8, 321, 8, params_old=[gb], params_new=[gb], precision=32)
+    assert _close_fraction(P, Po) > 0.97
+    assert abs(P.mean() - Po.mean()) < 2e-3 * abs(Po.mean()) + 1e-5
+    gimg = np.random.default_rng(14).uniform(0.5, 1.5, (10, 12, 3))
+    x = torch.from_numpy(gb).cuda().requires_grad_(True)
+    for _ in range(2):
+        p[key] = x
+        p.update()
+    g_gpu = plain.adjoint(sc, p, torch.from_numpy(gimg.astype(np.float32)).cuda(), 0, 17, 8)[key].cpu().numpy()
+    g_cpu = R.adjoint(sc.desc, 8, 8, 17, 8, gimg, params=[gb], precision=32)
+    scale = np.abs(g_cpu).max()
+    close = np.abs(g_gpu - g_cpu) <= 1e-3 * scale + 1e-3 * np.abs(g_cpu)
+    assert close.mean() > 0.97, close.mean()
+    assert abs(g_gpu.sum() - g_cpu.sum()) < 0.03 * np.abs(g_cpu).sum()
+
+
 def test_equal_states_bitwise_and_plain(cuda):
     announce("volume H == F == plain")
     g = _grid(3)
