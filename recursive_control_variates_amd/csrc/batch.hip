@@ -512,7 +512,7 @@ extern "C" int rcv_batch_add_job(rcv_batch *b, const rcv_batch_job *d, int32_t *
   J.cv_log_b2 = log(double(d->cv_beta2));
   J.pre_update = d->pre_update;
   J.grad = d->grad;
-  for (int q = 0; q < kMaxParams; ++q) J.gt.g[q] = nullptr;
+  J.gt.clear();
   J.gt.g[d->param] = d->grad;
   J.m = d->adam_m;
   J.v = d->adam_v;

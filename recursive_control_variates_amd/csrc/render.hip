@@ -227,6 +227,7 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
   VolPath<1, true, F> t;
   t.L[0] = v3(0.f);
   t.init_hom();
+  if (G.log_rec && (threadIdx.x & 63u) == 0) *wave_log_count(stack) = 0u;
   bool live = false, want_replay = false;
   int s = 0, s_end = 0, px = 0, py = 0;
   int64_t pixel = 0;
@@ -266,6 +267,55 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
     }
   }
   if constexpr ((F & kFeatHomog) != 0) flush_homogeneous(S, G, t.hom);
+  // the whole wavefront is past its loop: its record count is final
+  if (G.log_rec && (threadIdx.x & 63u) == 0)
+    G.log_count[(blockIdx.x * blockDim.x + threadIdx.x) >> 6] = min(*wave_log_count(stack), G.log_cap);
+}
+
+// Adds the scatter log of a persistent adjoint launch to the gradient of
+// an nx x ny x nz grid: block (x, y) takes tile y (voxels [y tile, (y + 1)
+// tile) in LDS) and the records of wavefronts x, x + gridDim.x, ..., adds
+// the trilinear weights of the record corners that fall in its tile (the
+// same products as scatter_grid), then adds the tile to the gradient. Each
+// record is read once per tile.
+__global__ __launch_bounds__(1024) void log_reduce_kernel(const int4 *__restrict__ rec, const float *__restrict__ val,
+                                                         const unsigned *__restrict__ count, int nwaves,
+                                                         unsigned cap, int nx, int nxy, int64_t nvox, int tile,
+                                                         float *__restrict__ g) {
+  extern __shared__ float acc[];
+  const int64_t lo = int64_t(blockIdx.y) * tile;
+  const int64_t hi = min(lo + int64_t(tile), nvox);
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) acc[i] = 0.0f;
+  __syncthreads();
+  const int64_t reach = int64_t(nxy) + nx + 1;  // largest corner offset from the base voxel
+  for (int w = blockIdx.x; w < nwaves; w += gridDim.x) {
+    const unsigned n = min(count[w], cap);
+    const int4 *r = rec + size_t(w) * cap;
+    const float *v = val + size_t(w) * cap;
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+      const int4 e = r[i];
+      const unsigned cell = unsigned(e.x);
+      const int64_t base = int64_t(cell & kRecBaseMask);
+      if (base + reach < lo || base >= hi) continue;
+      GridFoot f;
+      f.base = int(base);
+      f.dx = int((cell >> 26) & 1u);
+      f.dy = ((cell >> 27) & 1u) ? nx : 0;
+      f.dz = ((cell >> 28) & 1u) ? nxy : 0;
+      f.tx = __int_as_float(e.y);
+      f.ty = __int_as_float(e.z);
+      f.tz = __int_as_float(e.w);
+      const float x = v[i];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int64_t idx = f.idx(c);
+        if (idx >= lo && idx < hi) atomicAdd(&acc[idx - lo], x * f.w(c));
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < hi - lo; i += blockDim.x)
+    if (acc[i] != 0.0f) atomicAdd(&g[lo + i], acc[i]);
 }
 
 // The per-sample integrators' adjoint (volpathsimple, surface path tracer) as
@@ -433,6 +483,71 @@ static unsigned resident_grid(const void *kern, int block) {
   const unsigned g = unsigned(std::max(1, per_cu) * std::max(1, cus));
   cache[key] = g;
   return g;
+}
+
+// Scatter log of the persistent volume adjoint: record regions of kLogCap
+// records per wavefront, one buffer per (device, stream) kept for the
+// process (launches on one stream run in order; other streams get their
+// own), grown when a launch has more wavefronts. The reduction tiles hold
+// up to 32768 voxels (128 KB of LDS, less where the device allows less),
+// and grids of more than kLogMaxTiles tiles keep the atomics.
+constexpr unsigned kLogCap = 49152;
+constexpr int kLogMaxTiles = 16;
+constexpr int kLogChunks = 64;
+
+struct LogBuf {
+  int4 *rec = nullptr;
+  float *val = nullptr;
+  unsigned *count = nullptr;
+  size_t waves = 0;
+};
+
+static int log_buffer(hipStream_t stream, size_t waves, LogBuf &out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, LogBuf> bufs;
+  int dev = 0;
+  RCV_TRY_HIP(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(mu);
+  LogBuf &b = bufs[std::make_pair(dev, stream)];
+  if (b.waves < waves) {
+    if (b.waves) {
+      // the stream may still be reading the old buffer
+      RCV_TRY_HIP(hipStreamSynchronize(stream), "scatter log: stream synchronize");
+      (void)hipFree(b.rec);
+      (void)hipFree(b.val);
+      (void)hipFree(b.count);
+      b = LogBuf{};
+    }
+    RCV_TRY_HIP(hipMalloc(&b.rec, waves * kLogCap * sizeof(int4)), "scatter log: records");
+    RCV_TRY_HIP(hipMalloc(&b.val, waves * kLogCap * sizeof(float)), "scatter log: values");
+    RCV_TRY_HIP(hipMalloc(&b.count, waves * sizeof(unsigned)), "scatter log: counts");
+    b.waves = waves;
+  }
+  out = b;
+  return RCV_OK;
+}
+
+// Voxels per reduction tile on the current device (a power of two).
+static int log_tile() {
+  static std::mutex mu;
+  static std::map<int, int> tiles;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = tiles.find(dev);
+  if (it != tiles.end()) return it->second;
+  int max_lds = 0;
+  (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  int tile = 32768;
+  while (tile > 1024 && size_t(tile) * sizeof(float) > size_t(max_lds)) tile >>= 1;
+  if (tile * sizeof(float) > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void *>(log_reduce_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(tile * sizeof(float))) != hipSuccess) {
+    (void)hipGetLastError();
+    tile = 16384;
+  }
+  tiles[dev] = tile;
+  return tile;
 }
 
 // Work-unit split of a persistent volume launch: the chunk count of the
@@ -617,7 +732,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
     if (int rc = check_simple(s, fn)) return rc;
   if (int rc = check_states(s, 1, fn)) return rc;
   GradTargets G;
-  for (int i = 0; i < kMaxParams; ++i) G.g[i] = nullptr;
+  G.clear();
   for (int i = 0; i < n_params; ++i) {
     RCV_REQUIRE(param_ids[i] >= 0 && param_ids[i] < int(s->params.size()), "%s: bad param id %d", fn, param_ids[i]);
     RCV_REQUIRE(grad_ptrs[i], "%s: null gradient buffer for param %d", fn, param_ids[i]);
@@ -643,6 +758,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
     hipLaunchKernelGGL(kern, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp, ucl2,
                        grad_img, G, s->work_queue, units);
   };
+  int log_rc = RCV_OK;  // a failed scatter-log allocation
   if (d->type == RCV_INTEGRATOR_VOLPATH_SIMPLE && !d->samples_per_thread)
     with_volume_features(s->features(), [&](auto f) {
       launch_sample_persistent(adjoint_sample_persistent_kernel<RCV_INTEGRATOR_VOLPATH_SIMPLE, decltype(f)::value>);
@@ -663,9 +779,47 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       const unsigned pgrid =
           std::min(resident_grid(reinterpret_cast<const void *>(adjoint_persistent_kernel<F>), block),
                    ceil_div(int64_t(units), block));
+      // the density gradient of the first heterogeneous medium that has one
+      // goes through the scatter log when its grid fits the reduction tiles
+      GradTargets GL = G;
+      int lp = -1;
+      int64_t nvox = 0;
+      const int tile = log_tile();
+      if constexpr (!kTapeAdjoint<F>) {
+        for (const MediumInfo &M : s->media) {
+          if (M.homogeneous || M.sigma_param < 0 || !G.g[M.sigma_param]) continue;
+          const Param &P = s->params[M.sigma_param];
+          const int64_t n = int64_t(P.nx) * P.ny * P.nz;
+          if (n <= int64_t(tile) * kLogMaxTiles && n < (int64_t(1) << kRecBaseBits)) {
+            lp = M.sigma_param;
+            nvox = n;
+          }
+          break;
+        }
+      }
+      const size_t waves = size_t(pgrid) * block / 64;
+      LogBuf lb;
+      if (lp >= 0) {
+        if (int rc = log_buffer(stream, waves, lb)) {
+          log_rc = rc;
+          return;
+        }
+        GL.log_rec = lb.rec;
+        GL.log_val = lb.val;
+        GL.log_count = lb.count;
+        GL.log_cap = kLogCap;
+        GL.log_param = lp;
+      }
       hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
       hipLaunchKernelGGL(adjoint_persistent_kernel<F>, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view,
-                         I, sensor, seed, spp, ucl2, grad_img, G, s->work_queue, units);
+                         I, sensor, seed, spp, ucl2, grad_img, GL, s->work_queue, units);
+      if (lp >= 0) {
+        const Param &P = s->params[lp];
+        const int tiles = int((nvox + tile - 1) / tile);
+        hipLaunchKernelGGL(log_reduce_kernel, dim3(kLogChunks, tiles), dim3(1024), size_t(tile) * sizeof(float),
+                           stream, lb.rec, lb.val, lb.count, int(waves), kLogCap, int(P.nx), int(P.nx) * int(P.ny),
+                           nvox, tile, G.g[lp]);
+      }
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH)
     with_render_features(s->features(), [&](auto f) {
@@ -677,6 +831,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   else
     hipLaunchKernelGGL((adjoint_kernel<RCV_INTEGRATOR_PATH, 0>), dim3(grid), dim3(block), stack_bytes(block), stream,
                        s->view, I, sensor, seed, spp, spt, cl2, grad_img, G);
+  if (log_rc != RCV_OK) return log_rc;
   return check_launch(fn);
 }
 
@@ -695,7 +850,7 @@ int launch_forward(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
               "%s: forward mode exists for the path-replay integrators (prb, prbvolpath) only", fn);
   if (int rc = check_states(s, 1, fn)) return rc;
   GradTargets T;
-  for (int i = 0; i < kMaxParams; ++i) T.g[i] = nullptr;
+  T.clear();
   for (int i = 0; i < n_params; ++i) {
     RCV_REQUIRE(param_ids[i] >= 0 && param_ids[i] < int(s->params.size()), "%s: bad param id %d", fn, param_ids[i]);
     RCV_REQUIRE(tangents[i], "%s: null tangent for param %d", fn, param_ids[i]);

@@ -32,6 +32,25 @@ struct IntegArgs {
 
 struct GradTargets {
   float *g[kMaxParams];
+  // Scatter log (the persistent volume adjoint): the replay pass appends
+  // each scatter-add into g[log_param] as a record (packed footprint and
+  // fractions in log_rec, value in log_val) to its wavefront's region of
+  // log_cap records, instead of adding it; each wavefront stores its record
+  // count in log_count, and log_reduce_kernel adds the records up. Null
+  // log_rec: scatter-adds are atomics.
+  int4 *log_rec;
+  float *log_val;
+  unsigned *log_count;
+  unsigned log_cap;
+  int log_param;
+  __host__ __device__ void clear() {
+    for (int i = 0; i < kMaxParams; ++i) g[i] = nullptr;
+    log_rec = nullptr;
+    log_val = nullptr;
+    log_count = nullptr;
+    log_cap = 0;
+    log_param = -1;
+  }
 };
 
 // Event counters of a launch (scene counting mode): each call adds n per
@@ -333,6 +352,41 @@ __device__ inline bool pack_cell(const GridFoot &f, int medium, unsigned &cell) 
 
 __device__ inline int cell_medium(unsigned cell) { return int(cell >> 29); }
 
+// Scatter log: the record count of the lane's wavefront, in the last LDS
+// slot of its lane 0 (kernels that log keep their NEE weights out of it).
+__device__ inline unsigned *wave_log_count(const LdsStack &L) {
+  return reinterpret_cast<unsigned *>(L.slots + (L.nslots - 1) * L.stride - int(threadIdx.x & 63u));
+}
+
+// Scatter-add of val over footprint f into the gradient of param: appended
+// to the wavefront's log region when that gradient is logged (the active
+// lanes take consecutive records, counted with one LDS atomic per
+// wavefront), and added with atomics otherwise, or past the region's
+// capacity, or for a footprint that does not pack.
+__device__ inline void scatter_density(const GradTargets *G, const LdsStack &L, int param, float *g,
+                                       const GridFoot &f, float val) {
+  if (G->log_rec == nullptr || param != G->log_param) {
+    scatter_grid(g, f, val);
+    return;
+  }
+  const uint64_t m = __ballot(1);
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned base = 0;
+  if (int(__lane_id()) == leader)
+    base = __hip_atomic_fetch_add(wave_log_count(L), unsigned(__popcll(m)), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
+  base = __shfl(base, leader, 64);
+  const unsigned r = base + unsigned(__popcll(m & __lanemask_lt()));
+  unsigned cell;
+  if (r < G->log_cap && pack_cell(f, 0, cell)) {
+    const size_t at = size_t((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * G->log_cap + r;
+    G->log_rec[at] = make_int4(int(cell), __float_as_int(f.tx), __float_as_int(f.ty), __float_as_int(f.tz));
+    G->log_val[at] = val;
+  } else {
+    scatter_grid(g, f, val);
+  }
+}
+
 // The medium of the records being read back: its gradient (or tangent)
 // buffer and the row and slice strides of its density grid, loaded again
 // only when a record names another medium.
@@ -469,7 +523,7 @@ __device__ inline bool shadow_step(const SceneView &S, Shadow<NS> &sh, SurfInt &
         if (fwd_sum) {
           *fwd_sum += -M.scale / mev.sn[NS - 1] * grid_eval1(g, mev.foot);
         } else {
-          scatter_grid(g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
+          scatter_density(G, stack, M.sigma_param, g, mev.foot, -gscale * M.scale / mev.sn[NS - 1]);
           cnt.add(kCntGradAtomics, 8);
         }
       }
@@ -812,7 +866,7 @@ struct VolPath {
     }
     if constexpr (kNeeInline) {
       if (!replay) {
-        if (neew.k < stack.nslots) stack.slot(neew.k) = __float_as_int(dot(adj.dL, c[0]));
+        if (neew.k < stack.nslots - 1) stack.slot(neew.k) = __float_as_int(dot(adj.dL, c[0]));
       } else if (!have_cs) {
         // past the slots: track the shadow ray again, now with its weight
         const float cs = dot(adj.dL, c[0]);
@@ -1134,9 +1188,9 @@ struct VolPath {
         }
       } else if (gd && lr != 0.0f) {
         if (act_null) {
-          if (mev.sn[0] != 0.0f) scatter_grid(gd, mev.foot, -lr * M.scale / mev.sn[0]);
+          if (mev.sn[0] != 0.0f) scatter_density(G, stack, M.sigma_param, gd, mev.foot, -lr * M.scale / mev.sn[0]);
         } else {
-          scatter_grid(gd, mev.foot, lr * M.scale / mev.st[0]);
+          scatter_density(G, stack, M.sigma_param, gd, mev.foot, lr * M.scale / mev.st[0]);
         }
         cnt.add(kCntGradAtomics, 8);
       }
@@ -1220,7 +1274,7 @@ struct VolPath {
         float cs_in = 0.0f;
         bool have_cs = false;
         if constexpr (kNeeInline) {
-          if (replay && neew.k < stack.nslots) {
+          if (replay && neew.k < stack.nslots - 1) {
             cs_in = __int_as_float(stack.slot(neew.k));
             have_cs = true;
           }
