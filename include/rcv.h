@@ -204,7 +204,11 @@ int rcv_render_twostates(const rcv_scene *scene,
 /* Adjoint: render_backward (plugins/cv_integrator.py:87-94, inherited PRB /
  * PRBVolpath adjoint on the NEW state). grad_img: (film_h, film_w, 3) =
  * dLoss/dimage. Gradients are ACCUMULATED (+=) into grad_ptrs[i], which has
- * the shape of parameter param_ids[i]. Path replay: nothing is taped. */
+ * the shape of parameter param_ids[i]. Path replay: nothing is taped. The
+ * volume path tracer's adjoint (samples_per_thread 0) logs density
+ * scatter-adds in a buffer that the library allocates on first use per
+ * (device, stream), about 4 GB, and keeps until the process ends; calls on
+ * one stream share it in stream order. */
 int rcv_render_backward(const rcv_scene *scene,
                         const rcv_integrator_desc *integrator, int32_t sensor,
                         uint32_t seed, int32_t spp, const float *grad_img,
