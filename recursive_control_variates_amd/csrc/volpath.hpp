@@ -353,9 +353,10 @@ __device__ inline bool pack_cell(const GridFoot &f, int medium, unsigned &cell) 
 __device__ inline int cell_medium(unsigned cell) { return int(cell >> 29); }
 
 // Scatter log: the record count of the lane's wavefront, in the last LDS
-// slot of its lane 0 (kernels that log keep their NEE weights out of it).
+// word of its lane 0 (word kLdsWordsPerThread - 1 for every scene: past the
+// stack, and past the NEE weights, which leave the last slot free).
 __device__ inline unsigned *wave_log_count(const LdsStack &L) {
-  return reinterpret_cast<unsigned *>(L.slots + (L.nslots - 1) * L.stride - int(threadIdx.x & 63u));
+  return reinterpret_cast<unsigned *>(L.base + (kLdsWordsPerThread - 1) * L.stride - int(threadIdx.x & 63u));
 }
 
 // Scatter-add of val over footprint f into the gradient of param: appended
