@@ -227,7 +227,7 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
   VolPath<1, true, F> t;
   t.L[0] = v3(0.f);
   t.init_hom();
-  if (G.log_rec && (threadIdx.x & 63u) == 0) *wave_log_count(stack) = 0u;
+  log_begin(G, stack);
   bool live = false, want_replay = false;
   int s = 0, s_end = 0, px = 0, py = 0;
   int64_t pixel = 0;
@@ -267,49 +267,54 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
     }
   }
   if constexpr ((F & kFeatHomog) != 0) flush_homogeneous(S, G, t.hom);
-  // the whole wavefront is past its loop: its record count is final
-  if (G.log_rec && (threadIdx.x & 63u) == 0)
-    G.log_count[(blockIdx.x * blockDim.x + threadIdx.x) >> 6] = min(*wave_log_count(stack), G.log_cap);
+  // the whole wavefront is past its loop
+  log_end(G, stack);
 }
 
 // Adds the scatter log of a persistent adjoint launch to the gradient of
 // an nx x ny x nz grid: block (x, y) takes tile y (voxels [y tile, (y + 1)
-// tile) in LDS) and the records of wavefronts x, x + gridDim.x, ..., adds
-// the trilinear weights of the record corners that fall in its tile (the
-// same products as scatter_grid), then adds the tile to the gradient. Each
-// record is read once per tile.
+// tile) in LDS) and chunks x, x + gridDim.x, ... of the three bins whose
+// records reach into it (tile y, the seams below and above it), adds the
+// trilinear weights of the record corners that fall in its tile (the same
+// products as scatter_grid), then adds the tile to the gradient. A record
+// is read once, or twice when it lies across a seam.
 __global__ __launch_bounds__(1024) void log_reduce_kernel(const int4 *__restrict__ rec, const float *__restrict__ val,
-                                                         const unsigned *__restrict__ count, int nwaves,
-                                                         unsigned cap, int nx, int nxy, int64_t nvox, int tile,
-                                                         float *__restrict__ g) {
+                                                         const unsigned *__restrict__ ctl,
+                                                         const unsigned *__restrict__ list,
+                                                         const unsigned *__restrict__ fill, unsigned nchunks, int nx,
+                                                         int nxy, int64_t nvox, int tile, float *__restrict__ g) {
   extern __shared__ float acc[];
-  const int64_t lo = int64_t(blockIdx.y) * tile;
+  const int y = int(blockIdx.y);
+  const int64_t lo = int64_t(y) * tile;
   const int64_t hi = min(lo + int64_t(tile), nvox);
   for (int i = threadIdx.x; i < tile; i += blockDim.x) acc[i] = 0.0f;
   __syncthreads();
-  const int64_t reach = int64_t(nxy) + nx + 1;  // largest corner offset from the base voxel
-  for (int w = blockIdx.x; w < nwaves; w += gridDim.x) {
-    const unsigned n = min(count[w], cap);
-    const int4 *r = rec + size_t(w) * cap;
-    const float *v = val + size_t(w) * cap;
-    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
-      const int4 e = r[i];
-      const unsigned cell = unsigned(e.x);
-      const int64_t base = int64_t(cell & kRecBaseMask);
-      if (base + reach < lo || base >= hi) continue;
-      GridFoot f;
-      f.base = int(base);
-      f.dx = int((cell >> 26) & 1u);
-      f.dy = ((cell >> 27) & 1u) ? nx : 0;
-      f.dz = ((cell >> 28) & 1u) ? nxy : 0;
-      f.tx = __int_as_float(e.y);
-      f.ty = __int_as_float(e.z);
-      f.tz = __int_as_float(e.w);
-      const float x = v[i];
+  for (int k = 0; k < 3; ++k) {
+    if (k == 1 && y == 0) continue;
+    const int b = k == 0 ? y : kLogMaxTiles + y - (k == 1 ? 1 : 0);
+    const unsigned nb = min(ctl[1 + b], nchunks);
+    for (unsigned j = blockIdx.x; j < nb; j += gridDim.x) {
+      const unsigned c = list[size_t(b) * nchunks + j];
+      const unsigned n = min(fill[c], kLogChunk);
+      const int4 *r = rec + size_t(c) * kLogChunk;
+      const float *v = val + size_t(c) * kLogChunk;
+      for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+        const int4 e = r[i];
+        const unsigned cell = unsigned(e.x);
+        GridFoot f;
+        f.base = int(cell & kRecBaseMask);
+        f.dx = int((cell >> 26) & 1u);
+        f.dy = ((cell >> 27) & 1u) ? nx : 0;
+        f.dz = ((cell >> 28) & 1u) ? nxy : 0;
+        f.tx = __int_as_float(e.y);
+        f.ty = __int_as_float(e.z);
+        f.tz = __int_as_float(e.w);
+        const float x = v[i];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int64_t idx = f.idx(c);
-        if (idx >= lo && idx < hi) atomicAdd(&acc[idx - lo], x * f.w(c));
+        for (int q = 0; q < 8; ++q) {
+          const int64_t idx = f.idx(q);
+          if (idx >= lo && idx < hi) atomicAdd(&acc[idx - lo], x * f.w(q));
+        }
       }
     }
   }
@@ -485,53 +490,51 @@ static unsigned resident_grid(const void *kern, int block) {
   return g;
 }
 
-// Scatter log of the persistent volume adjoint: record regions of kLogCap
-// records per wavefront, one buffer per (device, stream) kept for the
+// Scatter log of the persistent volume adjoint: a pool of kLogPool chunks
+// of kLogChunk records (4 GB), one per (device, stream), kept for the
 // process (launches on one stream run in order; other streams get their
-// own), grown when a launch has more wavefronts. The reduction tiles hold
-// up to 32768 voxels (128 KB of LDS, less where the device allows less),
-// and grids of more than kLogMaxTiles tiles keep the atomics.
-constexpr unsigned kLogCap = 49152;
-constexpr int kLogMaxTiles = 16;
-constexpr int kLogChunks = 64;
+// own). The reduction tiles hold up to 32768 voxels (128 KB of LDS, less
+// where the device allows less); grids of more than kLogMaxTiles tiles, or
+// whose footprints span more than two tiles, keep the atomics. kLogSplits
+// blocks share each tile's reduction.
+constexpr unsigned kLogPool = 196608;
+constexpr int kLogSplits = 64;
 
 struct LogBuf {
   int4 *rec = nullptr;
   float *val = nullptr;
-  unsigned *count = nullptr;
-  size_t waves = 0;
+  unsigned *ctl = nullptr;   // 1 + kLogBins counters
+  unsigned *list = nullptr;  // kLogBins x kLogPool chunk indices
+  unsigned *fill = nullptr;  // kLogPool record counts
 };
 
-static int log_buffer(hipStream_t stream, size_t waves, LogBuf &out) {
+static int log_buffer(hipStream_t stream, LogBuf &out) {
   static std::mutex mu;
   static std::map<std::pair<int, hipStream_t>, LogBuf> bufs;
   int dev = 0;
   RCV_TRY_HIP(hipGetDevice(&dev), "hipGetDevice");
   std::lock_guard<std::mutex> lock(mu);
-  LogBuf &b = bufs[std::make_pair(dev, stream)];
-  if (b.waves < waves) {
-    if (b.waves) {
-      // the stream may still be reading the old buffer
-      RCV_TRY_HIP(hipStreamSynchronize(stream), "scatter log: stream synchronize");
-      (void)hipFree(b.rec);
-      (void)hipFree(b.val);
-      (void)hipFree(b.count);
-      b = LogBuf{};
-    }
+  const auto key = std::make_pair(dev, stream);
+  auto it = bufs.find(key);
+  if (it == bufs.end()) {
     LogBuf nb;
-    hipError_t e = hipMalloc(&nb.rec, waves * kLogCap * sizeof(int4));
-    if (e == hipSuccess) e = hipMalloc(&nb.val, waves * kLogCap * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&nb.count, waves * sizeof(unsigned));
+    const size_t recs = size_t(kLogPool) * kLogChunk;
+    hipError_t e = hipMalloc(&nb.rec, recs * sizeof(int4));
+    if (e == hipSuccess) e = hipMalloc(&nb.val, recs * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&nb.ctl, (1 + kLogBins) * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&nb.list, size_t(kLogBins) * kLogPool * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&nb.fill, size_t(kLogPool) * sizeof(unsigned));
     if (e != hipSuccess) {
       (void)hipFree(nb.rec);
       (void)hipFree(nb.val);
-      (void)hipFree(nb.count);
+      (void)hipFree(nb.ctl);
+      (void)hipFree(nb.list);
+      (void)hipFree(nb.fill);
       return hip_fail(e, "scatter log: allocation");
     }
-    nb.waves = waves;
-    b = nb;
+    it = bufs.emplace(key, nb).first;
   }
-  out = b;
+  out = it->second;
   return RCV_OK;
 }
 
@@ -791,6 +794,8 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
                    ceil_div(int64_t(units), block));
       // the density gradient of the first heterogeneous medium that has one
       // goes through the scatter log when its grid fits the reduction tiles
+      // and a footprint (up to nx ny + nx + 1 voxels past its base) spans
+      // at most two of them
       GradTargets GL = G;
       int lp = -1;
       int64_t nvox = 0;
@@ -800,26 +805,30 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
           if (M.homogeneous || M.sigma_param < 0 || !G.g[M.sigma_param]) continue;
           const Param &P = s->params[M.sigma_param];
           const int64_t n = int64_t(P.nx) * P.ny * P.nz;
-          if (n <= int64_t(tile) * kLogMaxTiles && n < (int64_t(1) << kRecBaseBits)) {
+          if (n <= int64_t(tile) * kLogMaxTiles && n < (int64_t(1) << kRecBaseBits) &&
+              int64_t(P.nx) * P.ny + P.nx + 1 <= tile) {
             lp = M.sigma_param;
             nvox = n;
           }
           break;
         }
       }
-      const size_t waves = size_t(pgrid) * block / 64;
       LogBuf lb;
       // without memory for the log the scatter-adds stay atomics
-      if (lp >= 0 && log_buffer(stream, waves, lb) != RCV_OK) {
+      if (lp >= 0 && log_buffer(stream, lb) != RCV_OK) {
         (void)hipGetLastError();
         lp = -1;
       }
       if (lp >= 0) {
         GL.log_rec = lb.rec;
         GL.log_val = lb.val;
-        GL.log_count = lb.count;
-        GL.log_cap = kLogCap;
+        GL.log_ctl = lb.ctl;
+        GL.log_list = lb.list;
+        GL.log_fill = lb.fill;
+        GL.log_chunks = kLogPool;
+        GL.log_shift = __builtin_ctz(unsigned(tile));
         GL.log_param = lp;
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1 + kLogBins), 0, stream, lb.ctl);
       }
       hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
       hipLaunchKernelGGL(adjoint_persistent_kernel<F>, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view,
@@ -827,9 +836,9 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       if (lp >= 0) {
         const Param &P = s->params[lp];
         const int tiles = int((nvox + tile - 1) / tile);
-        hipLaunchKernelGGL(log_reduce_kernel, dim3(kLogChunks, tiles), dim3(1024), size_t(tile) * sizeof(float),
-                           stream, lb.rec, lb.val, lb.count, int(waves), kLogCap, int(P.nx), int(P.nx) * int(P.ny),
-                           nvox, tile, G.g[lp]);
+        hipLaunchKernelGGL(log_reduce_kernel, dim3(kLogSplits, tiles), dim3(1024), size_t(tile) * sizeof(float),
+                           stream, lb.rec, lb.val, lb.ctl, lb.list, lb.fill, kLogPool, int(P.nx),
+                           int(P.nx) * int(P.ny), nvox, tile, G.g[lp]);
       }
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH)

@@ -30,25 +30,42 @@ struct IntegArgs {
   int flags;  // integrator-specific switches (volpathsimple: kSimpleNo*)
 };
 
+// Scatter log geometry: reduction tiles of a logged gradient, bins (one per
+// tile, then one per seam between tiles t and t + 1), records per chunk.
+constexpr int kLogMaxTiles = 16;
+constexpr int kLogBins = 2 * kLogMaxTiles;
+constexpr unsigned kLogChunk = 1024;
+
 struct GradTargets {
   float *g[kMaxParams];
   // Scatter log (the persistent volume adjoint): the replay pass appends
   // each scatter-add into g[log_param] as a record (packed footprint and
-  // fractions in log_rec, value in log_val) to its wavefront's region of
-  // log_cap records, instead of adding it; each wavefront stores its record
-  // count in log_count, and log_reduce_kernel adds the records up. Null
-  // log_rec: scatter-adds are atomics.
+  // fractions in log_rec, value in log_val) instead of adding it. Records
+  // are binned by the reduction tile (2^log_shift voxels) their corners
+  // fall in: bin t for tile t, bin kLogMaxTiles + t for footprints across
+  // tiles t and t + 1. A wavefront fills chunks of kLogChunk records, one
+  // per bin at a time, drawn from a pool of log_chunks: log_ctl[0] counts
+  // the chunks drawn, log_ctl[1 + b] those of bin b, listed in
+  // log_list[b * log_chunks + k]; log_fill[c] is chunk c's record count.
+  // log_reduce_kernel adds the records up. Null log_rec: scatter-adds are
+  // atomics.
   int4 *log_rec;
   float *log_val;
-  unsigned *log_count;
-  unsigned log_cap;
+  unsigned *log_ctl;
+  unsigned *log_list;
+  unsigned *log_fill;
+  unsigned log_chunks;
+  int log_shift;
   int log_param;
   __host__ __device__ void clear() {
     for (int i = 0; i < kMaxParams; ++i) g[i] = nullptr;
     log_rec = nullptr;
     log_val = nullptr;
-    log_count = nullptr;
-    log_cap = 0;
+    log_ctl = nullptr;
+    log_list = nullptr;
+    log_fill = nullptr;
+    log_chunks = 0;
+    log_shift = 0;
     log_param = -1;
   }
 };
@@ -352,39 +369,89 @@ __device__ inline bool pack_cell(const GridFoot &f, int medium, unsigned &cell) 
 
 __device__ inline int cell_medium(unsigned cell) { return int(cell >> 29); }
 
-// Scatter log: the record count of the lane's wavefront, in the last LDS
-// word of its lane 0 (word kLdsWordsPerThread - 1 for every scene: past the
-// stack, and past the NEE weights, which leave the last slot free).
-__device__ inline unsigned *wave_log_count(const LdsStack &L) {
-  return reinterpret_cast<unsigned *>(L.base + (kLdsWordsPerThread - 1) * L.stride - int(threadIdx.x & 63u));
+// Scatter log state of the lane's wavefront, in LDS word
+// kLdsWordsPerThread - 1 of its lanes (free in every scene: past the stack,
+// and past the NEE weights, which leave the last slot free): lane b holds
+// the chunk the wavefront fills for bin b (-1: none), lane kLogBins + b
+// that chunk's record count. Only the wavefront itself touches them.
+static_assert(2 * kLogBins == 64, "scatter log state: two words per bin in a wavefront's 64 lanes");
+
+__device__ inline int *wave_log_word(const LdsStack &L, int lane) {
+  return L.base + (kLdsWordsPerThread - 1) * L.stride - int(threadIdx.x & 63u) + lane;
 }
 
-// Scatter-add of val over footprint f into the gradient of param: appended
-// to the wavefront's log region when that gradient is logged (the active
-// lanes take consecutive records, counted with one LDS atomic per
-// wavefront), and added with atomics otherwise, or past the region's
-// capacity, or for a footprint that does not pack.
+// Before the wavefront's first scatter: no chunk in any bin.
+__device__ inline void log_begin(const GradTargets &G, const LdsStack &L) {
+  const int lane = int(threadIdx.x & 63u);
+  if (G.log_rec) *wave_log_word(L, lane) = lane < kLogBins ? -1 : 0;
+}
+
+// With the whole wavefront past its last scatter: the record counts of the
+// chunks it still fills.
+__device__ inline void log_end(const GradTargets &G, const LdsStack &L) {
+  const int lane = int(threadIdx.x & 63u);
+  if (G.log_rec && lane < kLogBins) {
+    const int c = *wave_log_word(L, lane);
+    if (c >= 0) G.log_fill[c] = unsigned(*wave_log_word(L, kLogBins + lane));
+  }
+}
+
+// Scatter-add of val over footprint f into the gradient of param. When
+// that gradient is logged the record goes to its bin's current chunk, at
+// the slot an LDS atomic on the bin's count gives the lane. Lanes left
+// without a slot (the chunk is full, or the bin has none yet) are rare:
+// for the bin of the first of them the chunk is closed (its count stored)
+// and the next one drawn from the pool with one global atomic, and those
+// of any other bin add with atomics this once. Added with atomics
+// otherwise, once the pool is used up, or for a footprint that does not
+// pack.
 __device__ inline void scatter_density(const GradTargets *G, const LdsStack &L, int param, float *g,
                                        const GridFoot &f, float val) {
-  if (G->log_rec == nullptr || param != G->log_param) {
+  unsigned cell;
+  if (G->log_rec == nullptr || param != G->log_param || !pack_cell(f, 0, cell)) {
     scatter_grid(g, f, val);
     return;
   }
-  const uint64_t m = __ballot(1);
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  unsigned base = 0;
-  if (int(__lane_id()) == leader)
-    base = __hip_atomic_fetch_add(wave_log_count(L), unsigned(__popcll(m)), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
-  base = __shfl(base, leader, 64);
-  const unsigned r = base + unsigned(__popcll(m & __lanemask_lt()));
-  unsigned cell;
-  if (r < G->log_cap && pack_cell(f, 0, cell)) {
-    const size_t at = size_t((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * G->log_cap + r;
-    G->log_rec[at] = make_int4(int(cell), __float_as_int(f.tx), __float_as_int(f.ty), __float_as_int(f.tz));
+  const int tile = f.base >> G->log_shift;
+  const int bin = ((f.base + f.dx + f.dy + f.dz) >> G->log_shift) == tile ? tile : kLogMaxTiles + tile;
+  const int4 rec = make_int4(int(cell), __float_as_int(f.tx), __float_as_int(f.ty), __float_as_int(f.tz));
+  const int chunk = *wave_log_word(L, bin);
+  const unsigned r = unsigned(__hip_atomic_fetch_add(wave_log_word(L, kLogBins + bin), 1, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WAVEFRONT));
+  const bool ok = chunk >= 0 && r < kLogChunk;
+  if (ok) {
+    const size_t at = size_t(chunk) * kLogChunk + r;
+    G->log_rec[at] = rec;
     G->log_val[at] = val;
-  } else {
-    scatter_grid(g, f, val);
+  }
+  const uint64_t bad = __ballot(!ok);
+  if (bad == 0) return;
+  const int lane = int(__lane_id());
+  const int leader = __ffsll((unsigned long long)bad) - 1;
+  const int b = __builtin_amdgcn_readlane(bin, leader);
+  const uint64_t mb = __ballot(!ok && bin == b);
+  int c = -1;
+  if (lane == leader) {
+    int *const cw = wave_log_word(L, b);
+    int *const fw = wave_log_word(L, kLogBins + b);
+    if (*cw >= 0) G->log_fill[*cw] = min(unsigned(*fw), kLogChunk);
+    const unsigned id = atomicAdd(G->log_ctl, 1u);
+    if (id < G->log_chunks) {
+      c = int(id);
+      G->log_list[size_t(b) * G->log_chunks + atomicAdd(G->log_ctl + 1 + b, 1u)] = id;
+    }
+    *cw = c;
+    *fw = c >= 0 ? int(__popcll(mb)) : 0;
+  }
+  c = __builtin_amdgcn_readlane(c, leader);
+  if (!ok) {
+    if (bin == b && c >= 0) {
+      const size_t at = size_t(c) * kLogChunk + unsigned(__popcll(mb & __lanemask_lt()));
+      G->log_rec[at] = rec;
+      G->log_val[at] = val;
+    } else {
+      scatter_grid(g, f, val);
+    }
   }
 }
 

@@ -294,6 +294,32 @@ def test_persistent_adjoint_equals_static_launch(cuda, kind):
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), np.abs(a - b).max() / scale
 
 
+def test_scatter_log_across_tiles_equals_atomics(cuda):
+    # The persistent volume adjoint logs the density scatter-adds and adds
+    # them up per 32768-voxel tile: a 48 x 48 x 40 grid spans three tiles
+    # whose boundaries cut through a slice, so records go to tile bins and to
+    # the seam bins read by two tiles. The static launch (samples_per_thread
+    # > 0) traces the same samples and adds the same products with atomics.
+    announce("scatter log over three tiles vs atomics (prbvolpath)")
+    rng = np.random.default_rng(21)
+    grid = (0.2 + rng.uniform(0, 0.8, (40, 48, 48, 1))).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(grid, albedo=0.8, scale=5.0, w=64, h=48, spp=32))
+    gimg = torch.from_numpy(rng.uniform(0.5, 1.5, (48, 64, 3)).astype(np.float32)).cuda()
+    integ = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    p = rcv.traverse(sc)
+    p[KEY] = p[KEY].detach().clone().requires_grad_(True)
+    p.update()
+    p.update()
+    grads = []
+    for spt in (0, 4):
+        integ.samples_per_thread = spt
+        grads.append(integ.adjoint(sc, p, gimg, 0, 23, 32)[KEY].double().cpu().numpy())
+    a, b = grads
+    scale = np.abs(b).max()
+    assert scale > 0 and (b != 0).mean() > 0.5, (b != 0).mean()
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), np.abs(a - b).max() / scale
+
+
 def _param_index(desc, key):
     return [p.key for p in desc.params].index(key)
 
