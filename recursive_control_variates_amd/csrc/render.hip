@@ -271,56 +271,152 @@ adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, i
   log_end(G, stack);
 }
 
-// Adds the scatter log of a persistent adjoint launch to the gradient of
-// an nx x ny x nz grid: block (x, y) takes tile y (voxels [y tile, (y + 1)
-// tile) in LDS) and chunks x, x + gridDim.x, ... of the three bins whose
-// records reach into it (tile y, the seams below and above it), adds the
-// trilinear weights of the record corners that fall in its tile (the same
-// products as scatter_grid), then adds the tile to the gradient. A record
-// is read once, or twice when it lies across a seam.
-__global__ __launch_bounds__(1024) void log_reduce_kernel(const int4 *__restrict__ rec, const float *__restrict__ val,
-                                                         const unsigned *__restrict__ ctl,
-                                                         const unsigned *__restrict__ list,
-                                                         const unsigned *__restrict__ fill, unsigned nchunks, int nx,
-                                                         int nxy, int64_t nvox, int tile, float *__restrict__ g) {
-  extern __shared__ float acc[];
+// Scatter log reduction. Block (x, y) takes tile y of the gradient
+// (voxels [y tile, (y + 1) tile)) and its share of the chunks of the three
+// bins whose records reach into it (tile y, the seams below and above it);
+// a record is read once, or twice when it lies across a seam. A block of
+// kReduceBlock threads takes kStepChunks chunks per step: thread i reads
+// slot i % kLogChunk of kReduceUnroll of them, all loads in flight together
+// (unused slots are read, every index stays inside the pool, and skipped).
+// The records of one chunk come from one wavefront of the adjoint, whose
+// lanes trace neighbouring pixels; the 8-lane groups of a wavefront take
+// their chunks in rotated order so that one instruction adds records of
+// several chunks.
+constexpr int kReduceBlock = 1024;
+constexpr int kReduceUnroll = 8;
+constexpr unsigned kChunkGroups = unsigned(kReduceBlock) / kLogChunk;
+constexpr unsigned kStepChunks = kChunkGroups * kReduceUnroll;
+static_assert(kReduceBlock % int(kLogChunk) == 0, "a reduction block covers whole chunks");
+
+// Calls fn(record, value) for each record of the block's share.
+template <bool REC, class Fn>
+__device__ inline void log_block_records(const int4 *__restrict__ rec, const float *__restrict__ val,
+                                         const unsigned *__restrict__ ctl, const unsigned *__restrict__ list,
+                                         const unsigned *__restrict__ fill, unsigned nchunks, Fn &&fn) {
   const int y = int(blockIdx.y);
-  const int64_t lo = int64_t(y) * tile;
-  const int64_t hi = min(lo + int64_t(tile), nvox);
-  for (int i = threadIdx.x; i < tile; i += blockDim.x) acc[i] = 0.0f;
-  __syncthreads();
+  const unsigned slot = threadIdx.x % kLogChunk;
+  const unsigned sub = threadIdx.x / kLogChunk;
+  const unsigned grp = (threadIdx.x >> 3) % kReduceUnroll;
   for (int k = 0; k < 3; ++k) {
     if (k == 1 && y == 0) continue;
     const int b = k == 0 ? y : kLogMaxTiles + y - (k == 1 ? 1 : 0);
     const unsigned nb = min(ctl[1 + b], nchunks);
-    for (unsigned j = blockIdx.x; j < nb; j += gridDim.x) {
-      const unsigned c = list[size_t(b) * nchunks + j];
-      const unsigned n = min(fill[c], kLogChunk);
-      const int4 *r = rec + size_t(c) * kLogChunk;
-      const float *v = val + size_t(c) * kLogChunk;
-      for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
-        const int4 e = r[i];
-        const unsigned cell = unsigned(e.x);
-        GridFoot f;
-        f.base = int(cell & kRecBaseMask);
-        f.dx = int((cell >> 26) & 1u);
-        f.dy = ((cell >> 27) & 1u) ? nx : 0;
-        f.dz = ((cell >> 28) & 1u) ? nxy : 0;
-        f.tx = __int_as_float(e.y);
-        f.ty = __int_as_float(e.z);
-        f.tz = __int_as_float(e.w);
-        const float x = v[i];
+    const unsigned *bl = list + size_t(b) * nchunks;
+    for (unsigned p0 = blockIdx.x * kStepChunks; p0 < nb; p0 += gridDim.x * kStepChunks) {
+      unsigned c[kReduceUnroll], n[kReduceUnroll];
+      int4 e[kReduceUnroll];
+      float x[kReduceUnroll];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int64_t idx = f.idx(q);
-          if (idx >= lo && idx < hi) atomicAdd(&acc[idx - lo], x * f.w(q));
-        }
+      for (int u = 0; u < kReduceUnroll; ++u) {
+        const unsigned j = p0 + sub + kChunkGroups * ((grp + unsigned(u)) % kReduceUnroll);
+        c[u] = j < nb ? bl[j] : 0u;
       }
+#pragma unroll
+      for (int u = 0; u < kReduceUnroll; ++u) {
+        const unsigned j = p0 + sub + kChunkGroups * ((grp + unsigned(u)) % kReduceUnroll);
+        n[u] = j < nb ? min(fill[c[u]], kLogChunk) : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kReduceUnroll; ++u) {
+        e[u] = REC ? rec[size_t(c[u]) * kLogChunk + slot] : make_int4(0, 0, 0, 0);
+        x[u] = val[size_t(c[u]) * kLogChunk + slot];
+      }
+#pragma unroll
+      for (int u = 0; u < kReduceUnroll; ++u)
+        if (slot < n[u]) fn(e[u], x[u]);
     }
   }
+}
+
+// Calls add(voxel - lo, value x weight) for the corners of a record that
+// fall in [lo, hi): the products of scatter_grid.
+template <class Add>
+__device__ inline void record_corners(const int4 &e, float x, int nx, int nxy, int64_t lo, int64_t hi, Add &&add) {
+  const unsigned cell = unsigned(e.x);
+  GridFoot f;
+  f.base = int(cell & kRecBaseMask);
+  f.dx = int((cell >> 26) & 1u);
+  f.dy = ((cell >> 27) & 1u) ? nx : 0;
+  f.dz = ((cell >> 28) & 1u) ? nxy : 0;
+  f.tx = __int_as_float(e.y);
+  f.ty = __int_as_float(e.z);
+  f.tz = __int_as_float(e.w);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t idx = f.idx(q);
+    if (idx >= lo && idx < hi) add(idx - lo, x * f.w(q));
+  }
+}
+
+// Adds the scatter log of a persistent adjoint launch to the gradient of
+// an nx x ny x nz grid, tile by tile (see log_block_records). The tile is
+// summed in LDS in 64-bit fixed point: integer LDS atomics run ~5x faster
+// than float ones here. A first pass over the share's values finds its
+// record count N and largest magnitude M; terms are scaled by 2^s with M N
+// 2^s <= 2^62, so no partial sum overflows and each term is kept to ~2^-44
+// of M (float atomics keep 2^-24 of the running sum). A share holding a
+// non-finite value is summed with float atomics as it stands.
+__global__ __launch_bounds__(kReduceBlock) void log_reduce_kernel(const int4 *__restrict__ rec,
+                                                              const float *__restrict__ val,
+                                                              const unsigned *__restrict__ ctl,
+                                                              const unsigned *__restrict__ list,
+                                                              const unsigned *__restrict__ fill, unsigned nchunks,
+                                                              int nx, int nxy, int64_t nvox, int tile,
+                                                              float *__restrict__ g) {
+  extern __shared__ long long acc[];  // tile words (floats in the first half for a non-finite share)
+  __shared__ float wave_max[kReduceBlock / 64];
+  __shared__ unsigned wave_count[kReduceBlock / 64];
+  const int64_t lo = int64_t(blockIdx.y) * tile;
+  const int64_t hi = min(lo + int64_t(tile), nvox);
+  float m = 0.0f;
+  unsigned total = 0;
+  log_block_records<false>(rec, val, ctl, list, fill, nchunks, [&](const int4 &, float x) {
+    const float a = fabsf(x);
+    m = (a <= m) ? m : (a == a ? a : INFINITY);  // NaN and inf both give inf
+    ++total;
+  });
+  for (int off = 32; off > 0; off >>= 1) {
+    m = fmaxf(m, __shfl_xor(m, off, 64));
+    total += __shfl_xor(total, off, 64);
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    wave_max[threadIdx.x >> 6] = m;
+    wave_count[threadIdx.x >> 6] = total;
+  }
   __syncthreads();
-  for (int64_t i = threadIdx.x; i < hi - lo; i += blockDim.x)
-    if (acc[i] != 0.0f) atomicAdd(&g[lo + i], acc[i]);
+  m = 0.0f;
+  total = 0;
+  for (int w = 0; w < kReduceBlock / 64; ++w) {
+    m = fmaxf(m, wave_max[w]);
+    total += wave_count[w];
+  }
+  if (total == 0 || m == 0.0f) return;
+  if (isfinite(m)) {
+    int em = 0, en = 0;
+    (void)frexpf(m, &em);               // m < 2^em
+    (void)frexpf(float(total), &en);    // total <= 2^en
+    const int sc = 62 - em - en;
+    for (int v = threadIdx.x; v < tile; v += blockDim.x) acc[v] = 0;
+    __syncthreads();
+    log_block_records<true>(rec, val, ctl, list, fill, nchunks, [&](const int4 &e, float x) {
+      record_corners(e, x, nx, nxy, lo, hi, [&](int64_t a, float t) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(&acc[a]), (unsigned long long)__float2ll_rn(ldexpf(t, sc)));
+      });
+    });
+    __syncthreads();
+    for (int64_t v = threadIdx.x; v < hi - lo; v += blockDim.x)
+      if (acc[v] != 0) atomicAdd(&g[lo + v], ldexpf(__ll2float_rn(acc[v]), -sc));
+  } else {
+    float *facc = reinterpret_cast<float *>(acc);
+    for (int v = threadIdx.x; v < tile; v += blockDim.x) facc[v] = 0.0f;
+    __syncthreads();
+    log_block_records<true>(rec, val, ctl, list, fill, nchunks, [&](const int4 &e, float x) {
+      record_corners(e, x, nx, nxy, lo, hi, [&](int64_t a, float t) { atomicAdd(&facc[a], t); });
+    });
+    __syncthreads();
+    for (int64_t v = threadIdx.x; v < hi - lo; v += blockDim.x)
+      if (facc[v] != 0.0f) atomicAdd(&g[lo + v], facc[v]);
+  }
 }
 
 // The per-sample integrators' adjoint (volpathsimple, surface path tracer) as
@@ -493,11 +589,11 @@ static unsigned resident_grid(const void *kern, int block) {
 // Scatter log of the persistent volume adjoint: a pool of kLogPool chunks
 // of kLogChunk records (4 GB), one per (device, stream), kept for the
 // process (launches on one stream run in order; other streams get their
-// own). The reduction tiles hold up to 32768 voxels (128 KB of LDS, less
+// own). The reduction tiles hold up to 16384 voxels (128 KB of LDS, less
 // where the device allows less); grids of more than kLogMaxTiles tiles, or
 // whose footprints span more than two tiles, keep the atomics. kLogSplits
 // blocks share each tile's reduction.
-constexpr unsigned kLogPool = 196608;
+constexpr unsigned kLogPool = 786432;
 constexpr int kLogSplits = 64;
 
 struct LogBuf {
@@ -538,10 +634,8 @@ static int log_buffer(hipStream_t stream, LogBuf &out) {
   return RCV_OK;
 }
 
-// Voxels per reduction tile on the current device: 32768 (128 KB of LDS),
-// or the largest power of two its LDS per block holds. (40960, 160 KB, was
-// no faster: 62.5 / 92.2 against 60.0 / 91.9 ms per adjoint at the mid /
-// final state.)
+// Voxels per reduction tile on the current device: 16384 (a 64-bit word
+// each, 128 KB of LDS), or the largest power of two its LDS per block holds.
 static int log_tile() {
   static std::mutex mu;
   static std::map<int, int> tiles;
@@ -552,13 +646,14 @@ static int log_tile() {
   if (it != tiles.end()) return it->second;
   int max_lds = 0;
   (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  int tile = 32768;
-  while (tile > 1024 && size_t(tile) * sizeof(float) > size_t(max_lds)) tile >>= 1;
-  if (tile * sizeof(float) > 65536 &&
+  constexpr size_t kStatic = 512;  // the kernel's static LDS, rounded up
+  int tile = 16384;
+  while (tile > 1024 && size_t(tile) * sizeof(long long) + kStatic > size_t(max_lds)) tile >>= 1;
+  if (tile * sizeof(long long) > 65536 &&
       hipFuncSetAttribute(reinterpret_cast<const void *>(log_reduce_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, int(tile * sizeof(float))) != hipSuccess) {
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(tile * sizeof(long long))) != hipSuccess) {
     (void)hipGetLastError();
-    tile = 16384;
+    tile = 65536 / int(sizeof(long long)) / 2;
   }
   tiles[dev] = tile;
   return tile;
@@ -836,7 +931,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       if (lp >= 0) {
         const Param &P = s->params[lp];
         const int tiles = int((nvox + tile - 1) / tile);
-        hipLaunchKernelGGL(log_reduce_kernel, dim3(kLogSplits, tiles), dim3(1024), size_t(tile) * sizeof(float),
+        hipLaunchKernelGGL(log_reduce_kernel, dim3(kLogSplits, tiles), dim3(kReduceBlock), size_t(tile) * sizeof(long long),
                            stream, lb.rec, lb.val, lb.ctl, lb.list, lb.fill, kLogPool, int(P.nx),
                            int(P.nx) * int(P.ny), nvox, tile, G.g[lp]);
       }

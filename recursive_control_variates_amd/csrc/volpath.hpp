@@ -34,7 +34,7 @@ struct IntegArgs {
 // tile, then one per seam between tiles t and t + 1), records per chunk.
 constexpr int kLogMaxTiles = 16;
 constexpr int kLogBins = 2 * kLogMaxTiles;
-constexpr unsigned kLogChunk = 1024;
+constexpr unsigned kLogChunk = 256;
 
 struct GradTargets {
   float *g[kMaxParams];

@@ -296,11 +296,12 @@ def test_persistent_adjoint_equals_static_launch(cuda, kind):
 
 def test_scatter_log_across_tiles_equals_atomics(cuda):
     # The persistent volume adjoint logs the density scatter-adds and adds
-    # them up per 32768-voxel tile: a 48 x 48 x 40 grid spans three tiles
-    # whose boundaries cut through a slice, so records go to tile bins and to
-    # the seam bins read by two tiles. The static launch (samples_per_thread
-    # > 0) traces the same samples and adds the same products with atomics.
-    announce("scatter log over three tiles vs atomics (prbvolpath)")
+    # them up per 16384-voxel tile (in 64-bit fixed point): a 48 x 48 x 40
+    # grid spans six tiles whose boundaries cut through a slice, so records
+    # go to tile bins and to the seam bins read by two tiles. The static
+    # launch (samples_per_thread > 0) traces the same samples and adds the
+    # same products with float atomics.
+    announce("scatter log over six tiles vs atomics (prbvolpath)")
     rng = np.random.default_rng(21)
     grid = (0.2 + rng.uniform(0, 0.8, (40, 48, 48, 1))).astype(np.float32)
     sc = rcv.load_dict(volume_box_scene(grid, albedo=0.8, scale=5.0, w=64, h=48, spp=32))
