@@ -321,6 +321,33 @@ def test_scatter_log_across_tiles_equals_atomics(cuda):
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), np.abs(a - b).max() / scale
 
 
+def test_scatter_log_non_finite_values_match_atomics(cuda):
+    # A tile whose records hold a NaN is summed with float atomics, as the
+    # static launch does: the same voxels come out NaN, the rest as before.
+    announce("scatter log with a NaN image gradient vs atomics (prbvolpath)")
+    rng = np.random.default_rng(22)
+    grid = (0.2 + rng.uniform(0, 0.8, (24, 24, 24, 1))).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(grid, albedo=0.8, scale=5.0, w=32, h=24, spp=32))
+    gimg_np = rng.uniform(0.5, 1.5, (24, 32, 3)).astype(np.float32)
+    gimg_np[12, 16, 1] = np.nan
+    gimg = torch.from_numpy(gimg_np).cuda()
+    integ = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    p = rcv.traverse(sc)
+    p[KEY] = p[KEY].detach().clone().requires_grad_(True)
+    p.update()
+    p.update()
+    grads = []
+    for spt in (0, 4):
+        integ.samples_per_thread = spt
+        grads.append(integ.adjoint(sc, p, gimg, 0, 29, 32)[KEY].double().cpu().numpy())
+    a, b = grads
+    assert np.isnan(b).any() and not np.isnan(b).all()
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    ok = ~np.isnan(b)
+    scale = np.abs(b[ok]).max()
+    assert np.allclose(a[ok], b[ok], rtol=1e-4, atol=1e-5 * scale), np.abs(a[ok] - b[ok]).max() / scale
+
+
 def _param_index(desc, key):
     return [p.key for p in desc.params].index(key)
 
