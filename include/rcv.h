@@ -207,7 +207,7 @@ int rcv_render_twostates(const rcv_scene *scene,
  * the shape of parameter param_ids[i]. Path replay: nothing is taped. The
  * volume path tracer's adjoint (samples_per_thread 0) logs density
  * scatter-adds in a buffer that the library allocates on first use per
- * (device, stream), about 4 GB, and keeps until the process ends; calls on
+ * (device, stream), about 3.3 GB, and keeps until the process ends; calls on
  * one stream share it in stream order. */
 int rcv_render_backward(const rcv_scene *scene,
                         const rcv_integrator_desc *integrator, int32_t sensor,

@@ -288,10 +288,10 @@ constexpr unsigned kChunkGroups = unsigned(kReduceBlock) / kLogChunk;
 constexpr unsigned kStepChunks = kChunkGroups * kReduceUnroll;
 static_assert(kReduceBlock % int(kLogChunk) == 0, "a reduction block covers whole chunks");
 
-// Calls fn(record, value) for each record of the block's share.
-template <bool REC, class Fn>
-__device__ inline void log_block_records(const int4 *__restrict__ rec, const float *__restrict__ val,
-                                         const unsigned *__restrict__ ctl, const unsigned *__restrict__ list,
+// Calls fn(record) for each record of the block's share.
+template <class Fn>
+__device__ inline void log_block_records(const int4 *__restrict__ rec, const unsigned *__restrict__ ctl,
+                                         const unsigned *__restrict__ list,
                                          const unsigned *__restrict__ fill, unsigned nchunks, Fn &&fn) {
   const int y = int(blockIdx.y);
   const unsigned slot = threadIdx.x % kLogChunk;
@@ -305,7 +305,6 @@ __device__ inline void log_block_records(const int4 *__restrict__ rec, const flo
     for (unsigned p0 = blockIdx.x * kStepChunks; p0 < nb; p0 += gridDim.x * kStepChunks) {
       unsigned c[kReduceUnroll], n[kReduceUnroll];
       int4 e[kReduceUnroll];
-      float x[kReduceUnroll];
 #pragma unroll
       for (int u = 0; u < kReduceUnroll; ++u) {
         const unsigned j = p0 + sub + kChunkGroups * ((grp + unsigned(u)) % kReduceUnroll);
@@ -317,30 +316,31 @@ __device__ inline void log_block_records(const int4 *__restrict__ rec, const flo
         n[u] = j < nb ? min(fill[c[u]], kLogChunk) : 0u;
       }
 #pragma unroll
-      for (int u = 0; u < kReduceUnroll; ++u) {
-        e[u] = REC ? rec[size_t(c[u]) * kLogChunk + slot] : make_int4(0, 0, 0, 0);
-        x[u] = val[size_t(c[u]) * kLogChunk + slot];
-      }
+      for (int u = 0; u < kReduceUnroll; ++u) e[u] = rec[size_t(c[u]) * kLogChunk + slot];
 #pragma unroll
       for (int u = 0; u < kReduceUnroll; ++u)
-        if (slot < n[u]) fn(e[u], x[u]);
+        if (slot < n[u]) fn(e[u]);
     }
   }
 }
 
-// Calls add(voxel - lo, value x weight) for the corners of a record that
-// fall in [lo, hi): the products of scatter_grid.
+// Calls add(voxel - lo, value x weight) for the corners of a record
+// (log_record) that fall in [lo, hi): the products of scatter_grid, with
+// the fractions as stored.
 template <class Add>
-__device__ inline void record_corners(const int4 &e, float x, int nx, int nxy, int64_t lo, int64_t hi, Add &&add) {
+__device__ inline void record_corners(const int4 &e, int nx, int nxy, int64_t lo, int64_t hi, Add &&add) {
   const unsigned cell = unsigned(e.x);
+  const unsigned long long q = (unsigned long long)unsigned(e.y) | ((unsigned long long)unsigned(e.z) << 32);
+  constexpr unsigned kMask = (1u << 21) - 1u;
   GridFoot f;
   f.base = int(cell & kRecBaseMask);
   f.dx = int((cell >> 26) & 1u);
   f.dy = ((cell >> 27) & 1u) ? nx : 0;
   f.dz = ((cell >> 28) & 1u) ? nxy : 0;
-  f.tx = __int_as_float(e.y);
-  f.ty = __int_as_float(e.z);
-  f.tz = __int_as_float(e.w);
+  f.tx = float(unsigned(q) & kMask) / kLogFracScale;
+  f.ty = float(unsigned(q >> 21) & kMask) / kLogFracScale;
+  f.tz = float(unsigned(q >> 42) & kMask) / kLogFracScale;
+  const float x = __int_as_float(e.w);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int64_t idx = f.idx(q);
@@ -357,7 +357,6 @@ __device__ inline void record_corners(const int4 &e, float x, int nx, int nxy, i
 // of M (float atomics keep 2^-24 of the running sum). A share holding a
 // non-finite value is summed with float atomics as it stands.
 __global__ __launch_bounds__(kReduceBlock) void log_reduce_kernel(const int4 *__restrict__ rec,
-                                                              const float *__restrict__ val,
                                                               const unsigned *__restrict__ ctl,
                                                               const unsigned *__restrict__ list,
                                                               const unsigned *__restrict__ fill, unsigned nchunks,
@@ -370,8 +369,8 @@ __global__ __launch_bounds__(kReduceBlock) void log_reduce_kernel(const int4 *__
   const int64_t hi = min(lo + int64_t(tile), nvox);
   float m = 0.0f;
   unsigned total = 0;
-  log_block_records<false>(rec, val, ctl, list, fill, nchunks, [&](const int4 &, float x) {
-    const float a = fabsf(x);
+  log_block_records(rec, ctl, list, fill, nchunks, [&](const int4 &e) {
+    const float a = fabsf(__int_as_float(e.w));
     m = (a <= m) ? m : (a == a ? a : INFINITY);  // NaN and inf both give inf
     ++total;
   });
@@ -398,8 +397,8 @@ __global__ __launch_bounds__(kReduceBlock) void log_reduce_kernel(const int4 *__
     const int sc = 62 - em - en;
     for (int v = threadIdx.x; v < tile; v += blockDim.x) acc[v] = 0;
     __syncthreads();
-    log_block_records<true>(rec, val, ctl, list, fill, nchunks, [&](const int4 &e, float x) {
-      record_corners(e, x, nx, nxy, lo, hi, [&](int64_t a, float t) {
+    log_block_records(rec, ctl, list, fill, nchunks, [&](const int4 &e) {
+      record_corners(e, nx, nxy, lo, hi, [&](int64_t a, float t) {
         atomicAdd(reinterpret_cast<unsigned long long *>(&acc[a]), (unsigned long long)__float2ll_rn(ldexpf(t, sc)));
       });
     });
@@ -410,8 +409,8 @@ __global__ __launch_bounds__(kReduceBlock) void log_reduce_kernel(const int4 *__
     float *facc = reinterpret_cast<float *>(acc);
     for (int v = threadIdx.x; v < tile; v += blockDim.x) facc[v] = 0.0f;
     __syncthreads();
-    log_block_records<true>(rec, val, ctl, list, fill, nchunks, [&](const int4 &e, float x) {
-      record_corners(e, x, nx, nxy, lo, hi, [&](int64_t a, float t) { atomicAdd(&facc[a], t); });
+    log_block_records(rec, ctl, list, fill, nchunks, [&](const int4 &e) {
+      record_corners(e, nx, nxy, lo, hi, [&](int64_t a, float t) { atomicAdd(&facc[a], t); });
     });
     __syncthreads();
     for (int64_t v = threadIdx.x; v < hi - lo; v += blockDim.x)
@@ -587,18 +586,17 @@ static unsigned resident_grid(const void *kern, int block) {
 }
 
 // Scatter log of the persistent volume adjoint: a pool of kLogPool chunks
-// of kLogChunk records (4 GB), one per (device, stream), kept for the
-// process (launches on one stream run in order; other streams get their
-// own). The reduction tiles hold up to 16384 voxels (128 KB of LDS, less
-// where the device allows less); grids of more than kLogMaxTiles tiles, or
-// whose footprints span more than two tiles, keep the atomics. kLogSplits
-// blocks share each tile's reduction.
+// of kLogChunk 16-byte records (3.2 GB, and 100 MB of chunk lists), one per
+// (device, stream), kept for the process (launches on one stream run in
+// order; other streams get their own). The reduction tiles hold up to
+// 16384 voxels (128 KB of LDS, less where the device allows less); grids of
+// more than kLogMaxTiles tiles, or whose footprints span more than two
+// tiles, keep the atomics. kLogSplits blocks share each tile's reduction.
 constexpr unsigned kLogPool = 786432;
 constexpr int kLogSplits = 64;
 
 struct LogBuf {
   int4 *rec = nullptr;
-  float *val = nullptr;
   unsigned *ctl = nullptr;   // 1 + kLogBins counters
   unsigned *list = nullptr;  // kLogBins x kLogPool chunk indices
   unsigned *fill = nullptr;  // kLogPool record counts
@@ -616,13 +614,11 @@ static int log_buffer(hipStream_t stream, LogBuf &out) {
     LogBuf nb;
     const size_t recs = size_t(kLogPool) * kLogChunk;
     hipError_t e = hipMalloc(&nb.rec, recs * sizeof(int4));
-    if (e == hipSuccess) e = hipMalloc(&nb.val, recs * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&nb.ctl, (1 + kLogBins) * sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&nb.list, size_t(kLogBins) * kLogPool * sizeof(unsigned));
     if (e == hipSuccess) e = hipMalloc(&nb.fill, size_t(kLogPool) * sizeof(unsigned));
     if (e != hipSuccess) {
       (void)hipFree(nb.rec);
-      (void)hipFree(nb.val);
       (void)hipFree(nb.ctl);
       (void)hipFree(nb.list);
       (void)hipFree(nb.fill);
@@ -916,7 +912,6 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       }
       if (lp >= 0) {
         GL.log_rec = lb.rec;
-        GL.log_val = lb.val;
         GL.log_ctl = lb.ctl;
         GL.log_list = lb.list;
         GL.log_fill = lb.fill;
@@ -932,7 +927,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
         const Param &P = s->params[lp];
         const int tiles = int((nvox + tile - 1) / tile);
         hipLaunchKernelGGL(log_reduce_kernel, dim3(kLogSplits, tiles), dim3(kReduceBlock), size_t(tile) * sizeof(long long),
-                           stream, lb.rec, lb.val, lb.ctl, lb.list, lb.fill, kLogPool, int(P.nx),
+                           stream, lb.rec, lb.ctl, lb.list, lb.fill, kLogPool, int(P.nx),
                            int(P.nx) * int(P.ny), nvox, tile, G.g[lp]);
       }
     });

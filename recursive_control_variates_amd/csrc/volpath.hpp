@@ -39,8 +39,8 @@ constexpr unsigned kLogChunk = 256;
 struct GradTargets {
   float *g[kMaxParams];
   // Scatter log (the persistent volume adjoint): the replay pass appends
-  // each scatter-add into g[log_param] as a record (packed footprint and
-  // fractions in log_rec, value in log_val) instead of adding it. Records
+  // each scatter-add into g[log_param] as a record in log_rec (log_record)
+  // instead of adding it. Records
   // are binned by the reduction tile (2^log_shift voxels) their corners
   // fall in: bin t for tile t, bin kLogMaxTiles + t for footprints across
   // tiles t and t + 1. A wavefront fills chunks of kLogChunk records, one
@@ -50,7 +50,6 @@ struct GradTargets {
   // log_reduce_kernel adds the records up. Null log_rec: scatter-adds are
   // atomics.
   int4 *log_rec;
-  float *log_val;
   unsigned *log_ctl;
   unsigned *log_list;
   unsigned *log_fill;
@@ -60,7 +59,6 @@ struct GradTargets {
   __host__ __device__ void clear() {
     for (int i = 0; i < kMaxParams; ++i) g[i] = nullptr;
     log_rec = nullptr;
-    log_val = nullptr;
     log_ctl = nullptr;
     log_list = nullptr;
     log_fill = nullptr;
@@ -369,6 +367,18 @@ __device__ inline bool pack_cell(const GridFoot &f, int medium, unsigned &cell) 
 
 __device__ inline int cell_medium(unsigned cell) { return int(cell >> 29); }
 
+// A scatter log record, 16 B: the packed footprint (pack_cell), the three
+// fractions as 21-bit fixed point (round((2^21 - 1) t); a fraction lies in
+// [0, 1]) in 63 bits, and the value.
+constexpr float kLogFracScale = 2097151.0f;
+
+__device__ inline int4 log_record(unsigned cell, const GridFoot &f, float val) {
+  const unsigned long long q = (unsigned long long)__float2uint_rn(f.tx * kLogFracScale) |
+                               ((unsigned long long)__float2uint_rn(f.ty * kLogFracScale) << 21) |
+                               ((unsigned long long)__float2uint_rn(f.tz * kLogFracScale) << 42);
+  return make_int4(int(cell), int(unsigned(q)), int(unsigned(q >> 32)), __float_as_int(val));
+}
+
 // Scatter log state of the lane's wavefront, in LDS word
 // kLdsWordsPerThread - 1 of its lanes (free in every scene: past the stack,
 // and past the NEE weights, which leave the last slot free): lane b holds
@@ -414,7 +424,7 @@ __device__ inline void scatter_density(const GradTargets *G, const LdsStack &L, 
   }
   const int tile = f.base >> G->log_shift;
   const int bin = ((f.base + f.dx + f.dy + f.dz) >> G->log_shift) == tile ? tile : kLogMaxTiles + tile;
-  const int4 rec = make_int4(int(cell), __float_as_int(f.tx), __float_as_int(f.ty), __float_as_int(f.tz));
+  const int4 rec = log_record(cell, f, val);
   const int chunk = *wave_log_word(L, bin);
   const unsigned r = unsigned(__hip_atomic_fetch_add(wave_log_word(L, kLogBins + bin), 1, __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_WAVEFRONT));
@@ -422,7 +432,6 @@ __device__ inline void scatter_density(const GradTargets *G, const LdsStack &L, 
   if (ok) {
     const size_t at = size_t(chunk) * kLogChunk + r;
     G->log_rec[at] = rec;
-    G->log_val[at] = val;
   }
   const uint64_t bad = __ballot(!ok);
   if (bad == 0) return;
@@ -448,7 +457,6 @@ __device__ inline void scatter_density(const GradTargets *G, const LdsStack &L, 
     if (bin == b && c >= 0) {
       const size_t at = size_t(c) * kLogChunk + unsigned(__popcll(mb & __lanemask_lt()));
       G->log_rec[at] = rec;
-      G->log_val[at] = val;
     } else {
       scatter_grid(g, f, val);
     }
