@@ -313,7 +313,8 @@ def rooflines(run, counts, window):
     first, count = window
     out = {}
     kinds = [("volume_adjoint", "adj_ms", "adjoint_bytes",
-              "adjoint_persistent_kernel<VOLPATH> (path replay)")]
+              "adjoint_persistent_kernel<VOLPATH> + log_reduce_kernel (path replay and its scatter-log "
+              "reduction, one rcv_render_backward call)")]
     if run["ts_ms"] is not None:
         kinds.append(("volume_twostate", "ts_ms", "twostate_bytes",
                       "render_persistent_kernel<2, VOLPATH> (two-state delta tracking)"))
