@@ -149,7 +149,7 @@ __global__ void batch_supergrid_kernel(const BatchJobDev *__restrict__ jobs, con
 
 // ---------------------------------------------------------------- 2. render
 template <int NS, int INTEG, int F>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void batch_render_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kWavesPerEuRender<NS, INTEG>))) void batch_render_kernel(const BatchJobDev *__restrict__ jobs, const int32_t *__restrict__ ids,
                                     const int64_t *__restrict__ ends, int n, int64_t total, unsigned *queue,
                                     int grab) {
   extern __shared__ int lds_stack[];

@@ -32,7 +32,7 @@ constexpr int kDefaultBlock = 256;  // threads per block when the descriptor say
 constexpr int kAdjUnitSamples = 16;  // camera samples per work unit of the persistent adjoint
 
 template <int NS, int INTEG, int F>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<INTEG>))) void render_kernel(
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEuRender<NS, INTEG>))) void render_kernel(
     SceneView S, IntegArgs I, int sensor, uint32_t seed,
                                                         int spp, int spt, int chunk_log2, float *img_old,
                                                         float *img_new) {
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
 // out, so no wave waits at the end of the launch for a few long pixels. The
 // result per sample is that of adjoint_kernel; only the order of the gradient
 // scatter-adds differs.
-template <int F>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH>))) void
+template <int F, int W>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(W))) void
 adjoint_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2,
                           const float *__restrict__ grad_img, GradTargets G, unsigned *queue, unsigned total_units) {
   extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
@@ -471,7 +471,7 @@ adjoint_sample_persistent_kernel(
 // `partial`; reduce_units_kernel then forms each pixel's mean from its units
 // in order, so the image does not depend on which lane took which unit.
 template <int NS, int F>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<RCV_INTEGRATOR_VOLPATH>))) void
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEuRender<NS, RCV_INTEGRATOR_VOLPATH>))) void
 render_persistent_kernel(SceneView S, IntegArgs I, int sensor, uint32_t seed, int spp, int chunk_log2,
                          float *__restrict__ partial, unsigned *queue, unsigned total_units) {
   extern __shared__ int lds_stack[];  // kLdsWordsPerThread words per thread
@@ -880,9 +880,6 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       constexpr int F = decltype(f)::value;
       const int ucl2 = unit_chunk_log2(spp, cl2);
       const unsigned units = unsigned(n_pix << ucl2);
-      const unsigned pgrid =
-          std::min(resident_grid(reinterpret_cast<const void *>(adjoint_persistent_kernel<F>), block),
-                   ceil_div(int64_t(units), block));
       // the density gradient of the first heterogeneous medium that has one
       // goes through the scatter log when its grid fits the reduction tiles
       // and a footprint (up to nx ny + nx + 1 voxels past its base) spans
@@ -920,9 +917,20 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
         GL.log_param = lp;
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1 + kLogBins), 0, stream, lb.ctl);
       }
-      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
-      hipLaunchKernelGGL(adjoint_persistent_kernel<F>, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view,
-                         I, sensor, seed, spp, ucl2, grad_img, GL, s->work_queue, units);
+      // three waves per SIMD with the log, four with atomics (kWavesPerEuLoggedAdjoint)
+      auto go = [&](auto kern) {
+        const unsigned pgrid =
+            std::min(resident_grid(reinterpret_cast<const void *>(kern), block), ceil_div(int64_t(units), block));
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
+        hipLaunchKernelGGL(kern, dim3(pgrid), dim3(block), stack_bytes(block), stream, s->view, I, sensor, seed, spp,
+                           ucl2, grad_img, GL, s->work_queue, units);
+      };
+      if constexpr (kTapeAdjoint<F>)
+        go(adjoint_persistent_kernel<F, kWavesPerEu<RCV_INTEGRATOR_VOLPATH>>);
+      else if (lp >= 0)
+        go(adjoint_persistent_kernel<F, kWavesPerEuLoggedAdjoint>);
+      else
+        go(adjoint_persistent_kernel<F, kWavesPerEu<RCV_INTEGRATOR_VOLPATH>>);
       if (lp >= 0) {
         const Param &P = s->params[lp];
         const int tiles = int((nvox + tile - 1) / tile);

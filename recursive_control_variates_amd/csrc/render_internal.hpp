@@ -50,6 +50,20 @@ inline void with_render_features(int f, Fn &&fn) {
 template <int INTEG>
 constexpr int kWavesPerEu = INTEG == RCV_INTEGRATOR_VOLPATH ? 4 : 3;
 
+// The volume path tracer's two-state render and its adjoint carry more
+// state (two throughputs and images; the replay's gradient terms): at four
+// waves they spilled 60 / 72 B per lane, and at three (up to 168 VGPRs, no
+// spills) they run 12-14 % / 17-18 % faster over the optimisation (two
+// waves: no faster than three). That holds for the adjoint whose density
+// gradient goes through the scatter log; adjoints that add with device-scope
+// atomics (batched, static, grids past the log) are faster at four, whose
+// extra wave covers the atomics' waits (the 64-job batch: 348 ms per
+// iteration against 376 ms with its adjoint at three). The single-state
+// render stays at four.
+template <int NS, int INTEG>
+constexpr int kWavesPerEuRender = INTEG == RCV_INTEGRATOR_VOLPATH && NS == 2 ? 3 : kWavesPerEu<INTEG>;
+constexpr int kWavesPerEuLoggedAdjoint = 3;
+
 // Splits the spp camera samples of a pixel over 2^chunk_log2 threads (at
 // most spt samples each): a user-given samples_per_thread must divide spp
 // into a power of two <= 64 chunks; otherwise chunks double, up to the
