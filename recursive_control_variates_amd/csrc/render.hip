@@ -14,6 +14,7 @@
 // adjoint's NEE weights): kLdsWordsPerThread words per thread in all.
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -630,6 +631,18 @@ static int log_buffer(hipStream_t stream, LogBuf &out) {
   return RCV_OK;
 }
 
+// Chunks of the pool a launch may draw: kLogPool, or fewer when the
+// environment sets RCV_SCATTER_LOG_CHUNKS (a test of the adjoint past the
+// pool's end, where the scatter-adds fall back to atomics).
+static unsigned log_pool_chunks() {
+  const char *e = std::getenv("RCV_SCATTER_LOG_CHUNKS");
+  if (!e || !*e) return kLogPool;
+  char *end = nullptr;
+  const unsigned long v = std::strtoul(e, &end, 10);
+  if (end == e || *end != '\0' || v == 0) return kLogPool;
+  return unsigned(std::min<unsigned long>(v, kLogPool));
+}
+
 // Voxels per reduction tile on the current device: 16384 (a 64-bit word
 // each, 128 KB of LDS), or the largest power of two its LDS per block holds.
 static int log_tile() {
@@ -912,7 +925,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
         GL.log_ctl = lb.ctl;
         GL.log_list = lb.list;
         GL.log_fill = lb.fill;
-        GL.log_chunks = kLogPool;
+        GL.log_chunks = log_pool_chunks();
         GL.log_shift = __builtin_ctz(unsigned(tile));
         GL.log_param = lp;
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1 + kLogBins), 0, stream, lb.ctl);
@@ -935,7 +948,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
         const Param &P = s->params[lp];
         const int tiles = int((nvox + tile - 1) / tile);
         hipLaunchKernelGGL(log_reduce_kernel, dim3(kLogSplits, tiles), dim3(kReduceBlock), size_t(tile) * sizeof(long long),
-                           stream, lb.rec, lb.ctl, lb.list, lb.fill, kLogPool, int(P.nx),
+                           stream, lb.rec, lb.ctl, lb.list, lb.fill, GL.log_chunks, int(P.nx),
                            int(P.nx) * int(P.ny), nvox, tile, G.g[lp]);
       }
     });

@@ -321,6 +321,32 @@ def test_scatter_log_across_tiles_equals_atomics(cuda):
     assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), np.abs(a - b).max() / scale
 
 
+@pytest.mark.parametrize("chunks", ["1", "40"])
+def test_scatter_log_past_the_pool_equals_atomics(cuda, monkeypatch, chunks):
+    # With the chunk pool capped (RCV_SCATTER_LOG_CHUNKS) the wavefronts draw
+    # it empty early in the launch; from then on their scatter-adds fall back
+    # to atomics, and the chunks they filled are still added up.
+    announce(f"scatter log with a pool of {chunks} chunks vs atomics (prbvolpath)")
+    monkeypatch.setenv("RCV_SCATTER_LOG_CHUNKS", chunks)
+    rng = np.random.default_rng(23)
+    grid = (0.2 + rng.uniform(0, 0.8, (40, 48, 48, 1))).astype(np.float32)
+    sc = rcv.load_dict(volume_box_scene(grid, albedo=0.8, scale=5.0, w=48, h=32, spp=32))
+    gimg = torch.from_numpy(rng.uniform(0.5, 1.5, (32, 48, 3)).astype(np.float32)).cuda()
+    integ = rcv.load_dict({"type": "prbvolpath", "max_depth": 8, "rr_depth": 8})
+    p = rcv.traverse(sc)
+    p[KEY] = p[KEY].detach().clone().requires_grad_(True)
+    p.update()
+    p.update()
+    grads = []
+    for spt in (0, 4):
+        integ.samples_per_thread = spt
+        grads.append(integ.adjoint(sc, p, gimg, 0, 37, 32)[KEY].double().cpu().numpy())
+    a, b = grads
+    scale = np.abs(b).max()
+    assert scale > 0 and (b != 0).mean() > 0.5, (b != 0).mean()
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), np.abs(a - b).max() / scale
+
+
 def test_scatter_log_non_finite_values_match_atomics(cuda):
     # A tile whose records hold a NaN is summed with float atomics, as the
     # static launch does: the same voxels come out NaN, the rest as before.
