@@ -30,7 +30,15 @@ namespace {
 
 constexpr int kMaxBlock = 256;      // launch bound of the render kernels
 constexpr int kDefaultBlock = 256;  // threads per block when the descriptor says 0
-constexpr int kAdjUnitSamples = 16;  // camera samples per work unit of the persistent adjoint
+constexpr int kAdjUnitSamples = 16;  // camera samples per work unit of the persistent per-sample adjoints
+// Camera samples per work unit of the volume path tracer's persistent
+// launches (the adjoint; the renders, within kRenderMaxUnitsPerPixel). With
+// small units the lanes of a wavefront trace samples of the same few pixels,
+// whose paths start alike and share L1 lines of the grid, and the last units
+// of a launch are short. The 64^3 adjoint at the mid / final state of the
+// optimisation: 46.1 / 67.5 ms with 16-sample units, 39.7 / 51.4 with 8,
+// 37.0 / 47.8 with 4, 36.2 / 46.4 with 2.
+constexpr int kVolUnitSamples = 2;
 
 template <int NS, int INTEG, int F>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEuRender<NS, INTEG>))) void render_kernel(
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kWave
 
 // Path-replay adjoint of the volume path tracer as a persistent launch: one
 // grid of resident waves whose lanes pull work units (a pixel's chunk of
-// kAdjUnitSamples camera samples) from a device counter, a wavefront's
+// kVolUnitSamples camera samples) from a device counter, a wavefront's
 // requests combined into one atomic, and keep tracing until the units run
 // out, so no wave waits at the end of the launch for a few long pixels. The
 // result per sample is that of adjoint_kernel; only the order of the gradient
@@ -670,10 +678,10 @@ static int log_tile() {
 
 // Work-unit split of a persistent volume launch: the chunk count of the
 // static launch (enough units to fill the device), raised while a unit keeps
-// at least kAdjUnitSamples samples.
-static int unit_chunk_log2(int spp, int static_cl2) {
+// at least min_samples samples.
+static int unit_chunk_log2(int spp, int static_cl2, int min_samples) {
   int ucl2 = 0;
-  while ((1 << ucl2) < 64 && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
+  while ((1 << ucl2) < 64 && (spp >> (ucl2 + 1)) >= min_samples) ++ucl2;
   return std::max(ucl2, static_cl2);
 }
 
@@ -802,7 +810,7 @@ int launch_render(const Scene *s, const rcv_integrator_desc *d, int sensor, uint
     // unit-sum buffer); smaller images that need more units per pixel to
     // fill the device take the static launch below
     int ucl2 = cl2;
-    while ((2 << ucl2) <= kRenderMaxUnitsPerPixel && (spp >> (ucl2 + 1)) >= kAdjUnitSamples) ++ucl2;
+    while ((2 << ucl2) <= kRenderMaxUnitsPerPixel && (spp >> (ucl2 + 1)) >= kVolUnitSamples) ++ucl2;
     const unsigned units = unsigned(n_pix << ucl2);
     RCV_REQUIRE(int64_t(units) * 3 * ns <= s->partial_cap, "%s: unit buffer too small", fn);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, s->work_queue);
@@ -868,7 +876,7 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
   // the per-sample integrators' persistent adjoint, as the volume path
   // tracer's below
   auto launch_sample_persistent = [&](auto kern) {
-    const int ucl2 = unit_chunk_log2(spp, cl2);
+    const int ucl2 = unit_chunk_log2(spp, cl2, kAdjUnitSamples);
     const unsigned units = unsigned(n_pix << ucl2);
     const unsigned pgrid =
         std::min(resident_grid(reinterpret_cast<const void *>(kern), block), ceil_div(int64_t(units), block));
@@ -888,10 +896,10 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
     });
   else if (d->type == RCV_INTEGRATOR_VOLPATH && !d->samples_per_thread)
     with_render_features(s->features(), [&](auto f) {
-      // the waves that fit, pulling units of >= kAdjUnitSamples samples
+      // the waves that fit, pulling units of >= kVolUnitSamples samples
       // (fewer where the image is too small to fill the device otherwise)
       constexpr int F = decltype(f)::value;
-      const int ucl2 = unit_chunk_log2(spp, cl2);
+      const int ucl2 = unit_chunk_log2(spp, cl2, kVolUnitSamples);
       const unsigned units = unsigned(n_pix << ucl2);
       // the density gradient of the first heterogeneous medium that has one
       // goes through the scatter log when its grid fits the reduction tiles
