@@ -899,7 +899,10 @@ int launch_adjoint(const Scene *s, const rcv_integrator_desc *d, int sensor, uin
       // the waves that fit, pulling units of >= kVolUnitSamples samples
       // (fewer where the image is too small to fill the device otherwise)
       constexpr int F = decltype(f)::value;
-      const int ucl2 = unit_chunk_log2(spp, cl2, kVolUnitSamples);
+      // (taped kernels keep kAdjUnitSamples: their per-lane tape in scratch
+      // gains nothing from coherence; the 64^3 k = 8 adjoint took 59.4 ms
+      // with 2-sample units against 51.2 ms)
+      const int ucl2 = unit_chunk_log2(spp, cl2, kTapeAdjoint<F> ? kAdjUnitSamples : kVolUnitSamples);
       const unsigned units = unsigned(n_pix << ucl2);
       // the density gradient of the first heterogeneous medium that has one
       // goes through the scatter log when its grid fits the reduction tiles
